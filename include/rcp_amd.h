@@ -1,0 +1,221 @@
+/*
+ * rcp_amd.h -- C ABI of the MI355X (gfx950) packing engine.
+ *
+ * This is the drop-in boundary for the hot path behind
+ * rcpgenerator.Packing.pack().  Each entry point names the reference
+ * interface it replaces (KD-physics/RCPGenerator, paths relative to
+ * python_code/python/).  Plain pointers and sizes only; host arrays are
+ * row-major [N][ndim] doubles exactly as the reference flattens them
+ * (src/core/rcp_generator.cpp:4359-4362).  All functions return 0 on success
+ * and a nonzero RCP_E* code otherwise; rcp_last_error() gives the message
+ * (thread-local).  Errors never cross this boundary as C++ exceptions.
+ *
+ * Numerics: FP64 everywhere.  The per-particle forces reproduce the
+ * reference's one-thread results bit for bit; the scalar reductions are
+ * deterministic and, with exact_reductions = 1, also bit-identical to the
+ * reference's one-thread summation order.
+ */
+#ifndef RCP_AMD_H
+#define RCP_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    RCP_OK = 0,
+    RCP_E_ARG = 1,        /* invalid argument (maps to ValueError) */
+    RCP_E_RUNTIME = 2,    /* reference-equivalent runtime_error (RuntimeError) */
+    RCP_E_HIP = 3,        /* HIP runtime failure / no GPU */
+    RCP_E_CALLBACK = 4,   /* the observer asked to stop (e.g. raised) */
+    RCP_E_NOMEM = 5
+};
+
+/* Message of the last failure on this thread ("" if none). */
+const char* rcp_last_error(void);
+
+/* Library build/device info: writes a short human-readable string. */
+int rcp_device_info(char* buf, size_t buflen);
+
+/* ------------------------------------------------------------------ runs
+ * Replaces rcpgenerator::PackingInput + PackingConfig + PackingRunOptions
+ * (include/rcpgenerator/rcp_generator.hpp:18-33, 102-123) as one struct.
+ */
+typedef struct rcp_run_config {
+    size_t N, ndim;
+    const double* positions;      /* N*ndim */
+    const double* diameters;      /* N */
+    const int8_t* fixed;          /* N or NULL: 1 = diameter frozen (cascade) */
+    const double* box;            /* nbox entries; nbox = 0 -> all ones */
+    size_t nbox;
+    const int8_t* walls;          /* nwalls entries; nwalls = 0 -> periodic */
+    size_t nwalls;
+    uint32_t neighbor_max;        /* per-particle capacity base (0 = default) */
+    uint32_t seed;                /* accepted, unused by the engine (as in the reference) */
+    int fix_height;
+    size_t max_steps;             /* 0 = schedule-derived cap */
+    int has_mu;              double mu;
+    int fix_diameter;
+    int has_target_phi;      double target_phi;
+    size_t reorder_interval;      /* reference default 2000; 0 disables */
+    int has_initial_phi_target; double initial_phi_target;
+    int exact_reductions;         /* 1 = reference one-thread summation order */
+} rcp_run_config;
+
+/* Fills the reference defaults (reorder_interval 2000, everything else 0). */
+void rcp_run_config_init(rcp_run_config* c);
+
+/* Replaces rcpgenerator::PackingObservation (rcp_generator.hpp:84-90). */
+typedef struct rcp_observation {
+    size_t step;
+    double phi, force_magnitude, energy, max_min_dist;
+} rcp_observation;
+
+/* Replaces rcpgenerator::PackingObserver (rcp_generator.hpp:125-127).
+ * positions is N*ndim or NULL.  Return nonzero to abort the run. */
+typedef int (*rcp_observer_fn)(void* user, const rcp_observation* obs,
+                               const double* positions);
+
+typedef struct rcp_engine rcp_engine;
+
+/* Set-up half of rcpgenerator::run_packing_observed (rcp_generator.cpp:4344-5013):
+ * wall normalisation, start-phi rescale, initial spatial reorder, first
+ * neighbour build and force evaluation, all device state resident in HBM. */
+int rcp_engine_create(const rcp_run_config* cfg, rcp_engine** out);
+
+/* Step loop of run_packing_observed (rcp_generator.cpp:5511-6662): runs at
+ * most nsteps further steps (0 = until the schedule terminates or the step
+ * cap).  progress_interval / capture_positions / trajectory_interval /
+ * observer as in the reference (rcp_generator.hpp:273-280). */
+int rcp_engine_run(rcp_engine* e, size_t nsteps, size_t progress_interval,
+                   int capture_positions, size_t trajectory_interval,
+                   rcp_observer_fn fn, void* user);
+
+/* Final observation / trace tail of run_packing_observed
+ * (rcp_generator.cpp:6737-6763); call once after the last rcp_engine_run. */
+int rcp_engine_finish(rcp_engine* e, int capture_positions, size_t trajectory_interval,
+                      size_t progress_interval, rcp_observer_fn fn, void* user);
+
+/* Blocks until all queued device work of the engine is complete. */
+int rcp_engine_sync(rcp_engine* e);
+
+/* Result shape (rcpgenerator::PackingResult, rcp_generator.hpp:35-82). */
+typedef struct rcp_result_info {
+    size_t N, ndim, steps;
+    double phi, max_min_dist, force_magnitude;
+    int done;                     /* schedule reached DONE (or the cap) */
+    size_t trace_len;             /* number of trace samples */
+} rcp_result_info;
+int rcp_engine_info(const rcp_engine* e, rcp_result_info* info);
+
+/* Copies the result arrays (any pointer may be NULL to skip):
+ * positions[N*ndim], diameters[N], particle_origin[N], box[ndim],
+ * walls[ndim], and the seven histories of length steps. */
+int rcp_engine_result(rcp_engine* e, double* positions, double* diameters,
+                      uint64_t* particle_origin, double* box, int8_t* walls,
+                      double* phi_history, double* force_history,
+                      double* energy_history, double* max_overlap_history,
+                      double* mu_history, double* alpha_history,
+                      int8_t* mu_flag_history);
+
+/* Trace sample k (rcpgenerator::PackingTrace, rcp_generator.hpp:92-100):
+ * step, phi, force, energy, max_min_dist and, when captured, positions
+ * [N*ndim] / diameters [N] (either pointer may be NULL).  has_positions
+ * reports whether the sample holds positions. */
+int rcp_engine_trace(const rcp_engine* e, size_t k, size_t* step, double* phi,
+                     double* force, double* energy, double* max_min_dist,
+                     double* positions, double* diameters, int* has_positions);
+
+/* Number of neighbour-list builds and total list entries so far
+ * (diagnostics: rebuild rate, mean row length). */
+int rcp_engine_stats(const rcp_engine* e, uint64_t* list_builds,
+                     uint64_t* list_entries, uint64_t* reorders);
+
+void rcp_engine_destroy(rcp_engine* e);
+
+/* One-call runs: rcpgenerator::run_packing / run_packing_observed
+ * (rcp_generator.hpp:269-280).  Results are read from the returned engine
+ * with rcp_engine_info / rcp_engine_result / rcp_engine_trace. */
+int rcp_run_packing_observed(const rcp_run_config* cfg, size_t progress_interval,
+                             int capture_positions, size_t trajectory_interval,
+                             rcp_observer_fn fn, void* user, rcp_engine** out);
+
+/* ------------------------------------------------------- kernel level
+ * Host-array wrappers around single device kernels, for parity tests and
+ * for callers that used the reference's exported kernels (rcp_generator.hpp
+ * 129-267).  Each copies its inputs to the GPU, runs, and copies back.
+ */
+
+/* get_forces_nd_3 (rcp_generator.hpp:207-235; rcp_generator.cpp:3597-4269)
+ * on positions x and diameters D with contacts found by the engine's own
+ * cell list.  F[N*ndim] out; scalars[0..5] = U, max_min_dist, Lc, Fmean,
+ * dkappa, count (finalised exactly as the reference).  rk may be NULL;
+ * sumD_active < 0 selects sum(D).  exact = 1 sums the scalars in the
+ * reference's one-thread order. */
+int rcp_forces(size_t N, size_t ndim, const double* x, const double* D,
+               const double* box, const int8_t* walls, double mu,
+               const double* rk, double sumD_active, int exact,
+               double* F, double* scalars);
+
+/* Full-refresh neighbour set of get_pairs_kdtree / get_pairs_nd_3
+ * (rcp_generator.cpp:2758-3526, 1198-1478; rcp_generator.hpp:185-205) by
+ * integer cell-hash lists.  Rows are ascending and hold j > i only.
+ * Call with nbr = NULL to get the count in *npairs; then with capacity.
+ * row_off has N+1 entries. */
+int rcp_pairs(size_t N, size_t ndim, const double* x, const double* D,
+              const double* box, uint64_t* row_off, uint32_t* nbr,
+              size_t nbr_capacity, size_t* npairs);
+
+/* compute_mean_force (rcp_generator.hpp:142-147; rcp_generator.cpp:769-803). */
+int rcp_mean_force(size_t N, size_t ndim, const double* F, double* out);
+
+/* The fused per-DOF ADAM + position + wrap pass (rcp_generator.cpp:6499-6520;
+ * adam_update rcp_generator.hpp:243-267 + position step).  m, v, x in/out. */
+int rcp_adam_positions(size_t N, size_t ndim, const double* F, double* m, double* v,
+                       double* x, const double* box, double alpha, double inv_b1,
+                       double inv_b2);
+
+/* Spatial-reorder permutation (rcp_generator.cpp:1084-1160): perm[k] = the
+ * old index placed at slot k. */
+int rcp_reorder_perm(size_t N, size_t ndim, const double* x, const double* box,
+                     uint32_t* perm);
+
+/* scale_diameters_nd (rcp_generator.hpp:171-176; rcp_generator.cpp:1162-1196),
+ * host arithmetic in the reference order. */
+int rcp_scale_diameters(size_t N, size_t ndim, const double* D, double phi_target,
+                        const double* box, int fix_height, double* out, double* factor);
+
+/* ------------------------------------------------------- initializer
+ * rcpgenerator::initialize_particles (include/rcpgenerator/initialize_particles.hpp:
+ * 18-50, 90; src/core/initialize_particles.cpp:399-474).  Host code (runs
+ * once per packing).  dist_type is one of mono, gaussian, bigaussian,
+ * bidisperse, lognormal, flat, powerlaw, exponential, weibull, custom. */
+typedef struct rcp_init_config {
+    double phi;
+    size_t N, ndim;
+    const double* box; size_t nbox;
+    const int8_t* walls; size_t nwalls;
+    int fix_height;
+    uint64_t seed;
+    const char* dist_type;
+    double d, mu, sigma, mu1, sigma1, mu2, sigma2, p, d1, d2, d_min, d_max,
+           exponent, scale, shape;
+    const double* custom; size_t ncustom;
+} rcp_init_config;
+
+/* Fills the reference Distribution / InitializerConfig defaults. */
+void rcp_init_config_init(rcp_init_config* c);
+
+/* Outputs: positions[N*ndim], diameters[N], box_out[ndim], walls_out[ndim],
+ * diameter_scale_factor, phi_modifier. */
+int rcp_initialize_particles(const rcp_init_config* c, double* positions,
+                             double* diameters, double* box_out, int8_t* walls_out,
+                             double* diameter_scale_factor, double* phi_modifier);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -372,6 +372,17 @@ double orc_shell(const double* D, size_t N)
     return mn * 0.5;
 }
 
+typedef struct { double x0; uint32_t idx; } walk_item;
+
+static int cmp_walk(const void* a, const void* b)
+{
+    const walk_item* p = (const walk_item*)a;
+    const walk_item* q = (const walk_item*)b;
+    if (p->x0 < q->x0) return -1;
+    if (p->x0 > q->x0) return 1;
+    return p->idx < q->idx ? -1 : (p->idx > q->idx);
+}
+
 size_t orc_pairs_full(size_t N, size_t ndim, const double* x, const double* D,
                       const double* box, uint64_t* row_off, uint32_t* nbr,
                       size_t cap)
@@ -380,28 +391,75 @@ size_t orc_pairs_full(size_t N, size_t ndim, const double* x, const double* D,
     double inv_box[ORC_MAXD];
     for (size_t d = 0; d < ndim; ++d) inv_box[d] = 1.0 / box[d];
     size_t used = 0;
+    if (ndim <= 3) {
+        /* kd-tree backend (RG:3276-3297): every pair passing the test once */
+        for (size_t i = 0; i < N; ++i) {
+            row_off[i] = used;
+            for (size_t j = i + 1; j < N; ++j) {
+                double d2 = 0.0;
+                for (size_t d = 0; d < ndim; ++d) {
+                    double dd = x[j * ndim + d] - x[i * ndim + d];
+                    dd -= nearbyint(dd * inv_box[d]) * box[d];
+                    d2 += dd * dd;
+                }
+                const double r_c = (D[i] + D[j]) * 0.5 + t;
+                if (d2 >= r_c * r_c) continue;
+                if (nbr && used < cap) nbr[used] = (uint32_t)j;
+                ++used;
+            }
+        }
+        row_off[N] = used;
+        return used;
+    }
+    /* sort+walk backend (RG:1215-1459) at full refresh: from each i walk the
+     * x-sorted order in both directions up to N/2 places, stopping at the
+     * first |dx| > r_c,max; own-write partners j > i without a duplicate
+     * check, then sort each row. */
+    double Dmax = D[0];
+    for (size_t i = 1; i < N; ++i) if (D[i] > Dmax) Dmax = D[i];
+    walk_item* w = (walk_item*)malloc((N + 1) * sizeof(walk_item));
+    size_t* loc = (size_t*)malloc((N + 1) * sizeof(size_t));
+    for (size_t i = 0; i < N; ++i) { w[i].x0 = x[i * ndim]; w[i].idx = (uint32_t)i; }
+    qsort(w, N, sizeof(walk_item), cmp_walk);
+    for (size_t k = 0; k < N; ++k) loc[w[k].idx] = k;
+    uint32_t* row = (uint32_t*)malloc((N + 1) * sizeof(uint32_t));
     for (size_t i = 0; i < N; ++i) {
         row_off[i] = used;
-        for (size_t j = i + 1; j < N; ++j) {
-            double d2 = 0.0;
-            for (size_t d = 0; d < ndim; ++d) {
-                double dd = x[j * ndim + d] - x[i * ndim + d];
-                dd -= nearbyint(dd * inv_box[d]) * box[d];
-                d2 += dd * dd;
+        size_t cnt = 0;
+        const double r_c_max = (D[i] + 1.01 * Dmax) / 2 + t;
+        const double Di_half_plus_t = D[i] * 0.5 + t;
+        for (int dir = -1; dir <= 1; dir += 2) {
+            long long delta = 0;
+            for (size_t jdx = 1; jdx <= N / 2; ++jdx) {
+                delta += dir;
+                long long pos = (long long)loc[i] + delta;
+                if (pos < 0) pos += (long long)N;
+                else if (pos >= (long long)N) pos -= (long long)N;
+                const size_t j = w[pos].idx;
+                double dx = x[j * ndim] - x[i * ndim];
+                dx -= nearbyint(dx * inv_box[0]) * box[0];
+                if (!(fabs(dx) <= r_c_max)) break;
+                const double r_c = Di_half_plus_t + (double)(float)(D[j] * 0.5);
+                if (fabs(dx) < r_c) {
+                    double d2 = dx * dx;
+                    for (size_t d = 1; d < ndim; ++d) {
+                        double dz = x[j * ndim + d] - x[i * ndim + d];
+                        dz -= nearbyint(dz * inv_box[d]) * box[d];
+                        d2 += dz * dz;
+                    }
+                    if (d2 >= r_c * r_c) continue;
+                    if (j > i) row[cnt++] = (uint32_t)j;
+                }
             }
-            double r_c;
-            if (ndim <= 3) {
-                r_c = (D[i] + D[j]) * 0.5 + t;                 /* RG:3295 */
-            } else {
-                double half_f = (double)(float)(D[j] * 0.5);    /* RG:1250-1256 */
-                r_c = (D[i] * 0.5 + t) + half_f;                /* RG:1357 */
-            }
-            if (d2 >= r_c * r_c) continue;
-            if (nbr && used < cap) nbr[used] = (uint32_t)j;
+        }
+        qsort(row, cnt, sizeof(uint32_t), cmp_u32);
+        for (size_t k = 0; k < cnt; ++k) {
+            if (nbr && used < cap) nbr[used] = row[k];
             ++used;
         }
     }
     row_off[N] = used;
+    free(w); free(loc); free(row);
     return used;
 }
 

@@ -1,0 +1,90 @@
+"""Build recipe for the in-tree native libraries (no cmake, no JIT cache).
+
+Outputs, next to the sources so they travel with the tree:
+  rcpgenerator_amd/lib/librcp_amd.so       HIP kernels (hipcc, gfx950) + host
+                                           engine + C ABI (g++)
+  rcpgenerator_amd/_rcpgenerator*.so        pybind11 module over the C ABI
+
+Device code is compiled with -ffp-contract=off (no fused multiply-add, so
+results round like the reference's portable build) and with packed FP32
+instructions disabled.  Host engine code is compiled with g++ -O3 without
+-march flags, matching the reference's portable arithmetic.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+import sysconfig
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INC = os.path.join(ROOT, "include")
+LIBDIR = os.path.join(PKG, "lib")
+BUILD = os.path.join(PKG, "_objs")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+ARCH = os.environ.get("RCP_OFFLOAD_ARCH", "gfx950")
+EXT_SUFFIX = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
+
+LIB_PATH = os.path.join(LIBDIR, "librcp_amd.so")
+MOD_PATH = os.path.join(PKG, "_rcpgenerator" + EXT_SUFFIX)
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I" + INC, "-I" + CSRC,
+]
+HOST_FLAGS = [
+    "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__",
+    "-I" + os.path.join(ROCM, "include"), "-I" + INC, "-I" + CSRC, "-Wall",
+]
+HOST_SOURCES = ["rcp_engine.cpp", "rcp_capi.cpp", "rcp_init.cpp"]
+HIP_SOURCES = ["rcp_kernels.hip"]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def _hipcc():
+    p = os.path.join(ROCM, "bin", "hipcc")
+    return p if os.path.exists(p) else (shutil.which("hipcc") or "hipcc")
+
+
+def build(verbose: bool = False) -> None:
+    """Compile every HIP/C++ source and link the library and the module."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+        _run([_hipcc(), *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
+        objs.append(obj)
+    for src in HOST_SOURCES:
+        obj = os.path.join(BUILD, src.replace(".cpp", ".o"))
+        _run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
+        objs.append(obj)
+    tmp = LIB_PATH + ".tmp"
+    _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
+          "-Wl,-soname,librcp_amd.so"], verbose)
+    os.replace(tmp, LIB_PATH)
+    import pybind11  # noqa: WPS433 (build-time only)
+    py_inc = sysconfig.get_paths()["include"]
+    tmp = MOD_PATH + ".tmp"
+    _run(["g++", "-O3", "-fPIC", "-shared", "-std=c++17", "-fvisibility=hidden",
+          "-I" + py_inc, "-I" + pybind11.get_include(), "-I" + INC,
+          os.path.join(CSRC, "rcp_module.cpp"), "-o", tmp,
+          "-L" + LIBDIR, "-lrcp_amd", "-Wl,-rpath,$ORIGIN/lib"], verbose)
+    os.replace(tmp, MOD_PATH)
+
+
+def built() -> bool:
+    return os.path.exists(LIB_PATH) and os.path.exists(MOD_PATH)
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)

@@ -1,0 +1,255 @@
+"""ctypes binding of the C ABI in include/rcp_amd.h.
+
+This is the binding a caller without pybind11 would add; the GPU parity
+tests call the engine through it so that every check crosses the same
+boundary a foreign-language host would use.  numpy arrays in, numpy arrays
+out; row-major [N][ndim] float64 positions as in the reference.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from ._build import LIB_PATH
+
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i8p = np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+
+RCP_OK, RCP_E_ARG, RCP_E_RUNTIME, RCP_E_HIP, RCP_E_CALLBACK, RCP_E_NOMEM = range(6)
+
+# Every exported symbol declared in include/rcp_amd.h.
+SYMBOLS = (
+    "rcp_last_error", "rcp_device_info", "rcp_run_config_init", "rcp_engine_create",
+    "rcp_engine_run", "rcp_engine_finish", "rcp_engine_sync", "rcp_engine_info",
+    "rcp_engine_result", "rcp_engine_trace", "rcp_engine_stats", "rcp_engine_destroy",
+    "rcp_run_packing_observed", "rcp_forces", "rcp_pairs", "rcp_mean_force",
+    "rcp_adam_positions", "rcp_reorder_perm", "rcp_scale_diameters", "rcp_init_config_init",
+    "rcp_initialize_particles",
+)
+
+
+class RunConfig(C.Structure):
+    _fields_ = [
+        ("N", C.c_size_t), ("ndim", C.c_size_t),
+        ("positions", C.c_void_p), ("diameters", C.c_void_p), ("fixed", C.c_void_p),
+        ("box", C.c_void_p), ("nbox", C.c_size_t),
+        ("walls", C.c_void_p), ("nwalls", C.c_size_t),
+        ("neighbor_max", C.c_uint32), ("seed", C.c_uint32), ("fix_height", C.c_int),
+        ("max_steps", C.c_size_t),
+        ("has_mu", C.c_int), ("mu", C.c_double),
+        ("fix_diameter", C.c_int),
+        ("has_target_phi", C.c_int), ("target_phi", C.c_double),
+        ("reorder_interval", C.c_size_t),
+        ("has_initial_phi_target", C.c_int), ("initial_phi_target", C.c_double),
+        ("exact_reductions", C.c_int),
+    ]
+
+
+class ResultInfo(C.Structure):
+    _fields_ = [
+        ("N", C.c_size_t), ("ndim", C.c_size_t), ("steps", C.c_size_t),
+        ("phi", C.c_double), ("max_min_dist", C.c_double), ("force_magnitude", C.c_double),
+        ("done", C.c_int), ("trace_len", C.c_size_t),
+    ]
+
+
+class RcpError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+def lib():
+    """Load librcp_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built")
+        L = C.CDLL(LIB_PATH)
+        L.rcp_last_error.restype = C.c_char_p
+        L.rcp_device_info.argtypes = [C.c_char_p, C.c_size_t]
+        L.rcp_run_config_init.argtypes = [C.POINTER(RunConfig)]
+        L.rcp_run_config_init.restype = None
+        L.rcp_engine_create.argtypes = [C.POINTER(RunConfig), C.POINTER(C.c_void_p)]
+        L.rcp_engine_run.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_int, C.c_size_t,
+                                     C.c_void_p, C.c_void_p]
+        L.rcp_engine_finish.argtypes = [C.c_void_p, C.c_int, C.c_size_t, C.c_size_t,
+                                        C.c_void_p, C.c_void_p]
+        L.rcp_engine_sync.argtypes = [C.c_void_p]
+        L.rcp_engine_info.argtypes = [C.c_void_p, C.POINTER(ResultInfo)]
+        L.rcp_engine_result.argtypes = [C.c_void_p] + [C.c_void_p] * 12
+        L.rcp_engine_stats.argtypes = [C.c_void_p, C.POINTER(C.c_uint64),
+                                       C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        L.rcp_engine_destroy.argtypes = [C.c_void_p]
+        L.rcp_engine_destroy.restype = None
+        L.rcp_forces.argtypes = [C.c_size_t, C.c_size_t, _dp, _dp, _dp, _i8p, C.c_double,
+                                 C.c_void_p, C.c_double, C.c_int, _dp, _dp]
+        L.rcp_pairs.argtypes = [C.c_size_t, C.c_size_t, _dp, _dp, _dp, _u64p, C.c_void_p,
+                                C.c_size_t, C.POINTER(C.c_size_t)]
+        L.rcp_mean_force.argtypes = [C.c_size_t, C.c_size_t, _dp, C.POINTER(C.c_double)]
+        L.rcp_adam_positions.argtypes = [C.c_size_t, C.c_size_t, _dp, _dp, _dp, _dp, _dp,
+                                         C.c_double, C.c_double, C.c_double]
+        L.rcp_reorder_perm.argtypes = [C.c_size_t, C.c_size_t, _dp, _dp, _u32p]
+        L.rcp_scale_diameters.argtypes = [C.c_size_t, C.c_size_t, _dp, C.c_double, _dp,
+                                          C.c_int, _dp, C.POINTER(C.c_double)]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != RCP_OK:
+        raise RcpError(rc, lib().rcp_last_error().decode())
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def device_info() -> str:
+    buf = C.create_string_buffer(256)
+    _check(lib().rcp_device_info(buf, 256))
+    return buf.value.decode()
+
+
+def forces(x, D, box, walls=None, mu=0.0, rk=None, sumD_active=-1.0, exact=True):
+    """rcp_forces: returns (F[N, ndim], dict of finalised scalars)."""
+    x, D, box = _f64(x), _f64(D), _f64(box)
+    N, nd = x.shape
+    w = np.ascontiguousarray(walls if walls is not None else np.zeros(nd), dtype=np.int8)
+    F = np.zeros((N, nd))
+    s = np.zeros(6)
+    rkp = None if rk is None else _f64(rk)
+    _check(lib().rcp_forces(N, nd, x, D, box, w, mu, None if rkp is None else rkp.ctypes.data,
+                            sumD_active, int(exact), F, s))
+    return F, {"U": s[0], "max_min_dist": s[1], "Lc": s[2], "Fmean": s[3], "dkappa": s[4],
+               "count": int(s[5])}
+
+
+def pairs(x, D, box):
+    """rcp_pairs: full-refresh half list as CSR (row_off[N+1], nbr[nnz])."""
+    x, D, box = _f64(x), _f64(D), _f64(box)
+    N, nd = x.shape
+    off = np.zeros(N + 1, dtype=np.uint64)
+    n = C.c_size_t(0)
+    _check(lib().rcp_pairs(N, nd, x, D, box, off, None, 0, C.byref(n)))
+    nbr = np.zeros(max(n.value, 1), dtype=np.uint32)
+    _check(lib().rcp_pairs(N, nd, x, D, box, off, nbr.ctypes.data, nbr.size, C.byref(n)))
+    return off, nbr[: n.value]
+
+
+def mean_force(F) -> float:
+    F = _f64(F)
+    out = C.c_double(0.0)
+    _check(lib().rcp_mean_force(F.shape[0], F.shape[1], F, C.byref(out)))
+    return out.value
+
+
+def adam_positions(F, m, v, x, box, alpha, inv_b1, inv_b2):
+    F, box = _f64(F), _f64(box)
+    m, v, x = _f64(m).copy(), _f64(v).copy(), _f64(x).copy()
+    _check(lib().rcp_adam_positions(x.shape[0], x.shape[1], F, m, v, x, box, alpha, inv_b1,
+                                    inv_b2))
+    return m, v, x
+
+
+def reorder_perm(x, box):
+    x, box = _f64(x), _f64(box)
+    perm = np.zeros(x.shape[0], dtype=np.uint32)
+    _check(lib().rcp_reorder_perm(x.shape[0], x.shape[1], x, box, perm))
+    return perm
+
+
+def scale_diameters(D, phi_target, box, ndim, fix_height=False):
+    D, box = _f64(D), _f64(box)
+    out = np.zeros_like(D)
+    f = C.c_double(0.0)
+    _check(lib().rcp_scale_diameters(D.shape[0], ndim, D, phi_target, box, int(fix_height), out,
+                                     C.byref(f)))
+    return out, f.value
+
+
+class Engine:
+    """Owning handle over an rcp_engine (step-wise runs through the C ABI)."""
+
+    def __init__(self, positions, diameters, box=None, walls=None, fixed=None, fix_height=False,
+                 max_steps=0, mu=None, fix_diameter=False, target_phi=None,
+                 reorder_interval=2000, initial_phi_target=None, exact=False, neighbor_max=0):
+        self._x = _f64(positions)
+        self.N, self.ndim = self._x.shape
+        self._D = _f64(diameters)
+        self._box = _f64(box if box is not None else [])
+        self._walls = np.ascontiguousarray(walls if walls is not None else [], dtype=np.int8)
+        self._fixed = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.int8)
+        c = RunConfig()
+        lib().rcp_run_config_init(C.byref(c))
+        c.N, c.ndim = self.N, self.ndim
+        c.positions, c.diameters = self._x.ctypes.data, self._D.ctypes.data
+        c.fixed = None if self._fixed is None else self._fixed.ctypes.data
+        c.box = self._box.ctypes.data if self._box.size else None
+        c.nbox = self._box.size
+        c.walls = self._walls.ctypes.data if self._walls.size else None
+        c.nwalls = self._walls.size
+        c.neighbor_max = neighbor_max
+        c.fix_height = int(fix_height)
+        c.max_steps = max_steps
+        c.has_mu, c.mu = int(mu is not None), float(mu or 0.0)
+        c.fix_diameter = int(fix_diameter)
+        c.has_target_phi, c.target_phi = int(target_phi is not None), float(target_phi or 0.0)
+        c.reorder_interval = reorder_interval
+        c.has_initial_phi_target = int(initial_phi_target is not None)
+        c.initial_phi_target = float(initial_phi_target or 0.55)
+        c.exact_reductions = int(exact)
+        self._cfg = c
+        h = C.c_void_p()
+        _check(lib().rcp_engine_create(C.byref(c), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().rcp_engine_destroy(h)
+            self._h = None
+
+    def run(self, nsteps: int = 0) -> None:
+        _check(lib().rcp_engine_run(self._h, nsteps, 0, 0, 0, None, None))
+
+    def finish(self) -> None:
+        _check(lib().rcp_engine_finish(self._h, 0, 0, 0, None, None))
+
+    def sync(self) -> None:
+        _check(lib().rcp_engine_sync(self._h))
+
+    def info(self) -> ResultInfo:
+        i = ResultInfo()
+        _check(lib().rcp_engine_info(self._h, C.byref(i)))
+        return i
+
+    def stats(self):
+        b, e, r = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(lib().rcp_engine_stats(self._h, C.byref(b), C.byref(e), C.byref(r)))
+        return {"list_builds": b.value, "list_entries": e.value, "reorders": r.value}
+
+    def result(self) -> dict:
+        i = self.info()
+        N, nd, s = i.N, i.ndim, i.steps
+        x = np.zeros((N, nd)); D = np.zeros(N); o = np.zeros(N, dtype=np.uint64)
+        box = np.zeros(nd); w = np.zeros(nd, dtype=np.int8)
+        H = [np.zeros(s) for _ in range(6)]
+        fl = np.zeros(s, dtype=np.int8)
+        _check(lib().rcp_engine_result(self._h, x.ctypes.data, D.ctypes.data, o.ctypes.data,
+                                       box.ctypes.data, w.ctypes.data,
+                                       *[h.ctypes.data for h in H], fl.ctypes.data))
+        keys = ("phi_history", "force_history", "energy_history", "max_overlap_history",
+                "mu_history", "alpha_history")
+        out = {"positions": x, "diameters": D, "particle_origin": o, "box": box, "walls": w,
+               "mu_flag_history": fl, "steps": s, "phi": i.phi,
+               "max_min_dist": i.max_min_dist, "force_magnitude": i.force_magnitude,
+               "done": bool(i.done)}
+        out.update(dict(zip(keys, H)))
+        return out

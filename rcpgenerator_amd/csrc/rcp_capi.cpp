@@ -1,0 +1,521 @@
+// C ABI (include/rcp_amd.h): engine lifecycle and kernel-level wrappers.
+// Exceptions never cross this boundary; they become RCP_E* codes with the
+// message kept in a thread-local string.
+#include "rcp_amd.h"
+#include "rcp_engine.h"
+#include "rcp_kernels.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+struct rcp_engine {
+    std::unique_ptr<rcp::Engine> e;
+};
+
+namespace {
+
+template <class F>
+int guarded(F&& f)
+{
+    try {
+        rcp::g_last_error.clear();
+        return f();
+    } catch (const rcp::ArgError& ex) {
+        rcp::g_last_error = ex.what();
+        return RCP_E_ARG;
+    } catch (const rcp::HipFailure& ex) {
+        rcp::g_last_error = ex.what();
+        return RCP_E_HIP;
+    } catch (const rcp::RtError& ex) {
+        rcp::g_last_error = ex.what();
+        return RCP_E_RUNTIME;
+    } catch (const std::bad_alloc&) {
+        rcp::g_last_error = "out of memory";
+        return RCP_E_NOMEM;
+    } catch (const std::exception& ex) {
+        rcp::g_last_error = ex.what();
+        return RCP_E_RUNTIME;
+    }
+}
+
+// RAII device buffer for the kernel-level wrappers.
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    explicit DBuf(size_t count) : n(count) { p = static_cast<T*>(rcp::dev_alloc(count * sizeof(T))); }
+    ~DBuf() { rcp::dev_free(p); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    void up(const T* h, size_t count, hipStream_t s)
+    {
+        if (count) rcp::hip_check(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+    }
+    void down(T* h, size_t count, hipStream_t s) const
+    {
+        if (count) rcp::hip_check(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s), "download");
+    }
+};
+
+struct Stream {
+    hipStream_t s = nullptr;
+    Stream() { rcp::hip_check(hipStreamCreate(&s), "hipStreamCreate"); }
+    ~Stream() { if (s) { (void)hipStreamSynchronize(s); (void)hipStreamDestroy(s); } }
+    void sync() const { rcp::hip_check(hipStreamSynchronize(s), "sync"); }
+};
+
+void check_dims(size_t N, size_t ndim)
+{
+    if (N == 0) throw rcp::ArgError("rcp: N must be >= 1");
+    if (ndim < 1 || ndim > (size_t)rcp::kMaxDim) throw rcp::ArgError("rcp: ndim must be in 1..8");
+    if (N > 0x7fffffffull) throw rcp::ArgError("rcp: N too large");
+}
+
+rcp::Geometry make_geometry(size_t N, size_t ndim, const double* box, const int8_t* walls)
+{
+    rcp::Geometry g;
+    g.N = (int)N;
+    g.ndim = (int)ndim;
+    for (size_t d = 0; d < ndim; ++d) {
+        g.box[d] = box[d];
+        g.inv_box[d] = 1.0 / box[d];
+        g.walls[d] = walls ? walls[d] : 0;
+    }
+    g.curved_T = (walls && walls[0] < 0) ? -walls[0] : 0;
+    if ((size_t)g.curved_T > ndim) throw rcp::ArgError("rcp: curved container rank exceeds ndim");
+    return g;
+}
+
+// Cell-list rows on the device for positions x (device) and diameters D0
+// (device, kappa = 1): full rows (half = 0) or j > i rows (half = 1).
+struct DeviceRows {
+    std::unique_ptr<DBuf<uint32_t>> cell_of, slot, items, row_count, row_start, cell_count,
+        cell_start, nbr;
+    std::unique_ptr<DBuf<char>> scan_tmp;
+    uint32_t nnz = 0;
+};
+
+void device_rows(const rcp::Geometry& g, const double* x, const double* D0, double Dmax,
+                 double t_shell, int half, DeviceRows& r, hipStream_t s)
+{
+    const size_t N = (size_t)g.N;
+    rcp::ListArgs la;
+    la.g = g;
+    la.x = x;
+    la.D0 = D0;
+    la.kappa = 1.0;
+    la.t_shell = t_shell;
+    la.half = half;
+    const double edge = (Dmax + t_shell) * (1.0 + 1e-6);
+    long long total = 1;
+    for (int d = 0; d < g.ndim; ++d) {
+        long long n = (edge > 0.0) ? (long long)std::floor(g.box[d] / edge) : 1;
+        if (n < 3) n = 1;
+        if (n > (1 << 20)) n = 1 << 20;
+        la.grid.nc[d] = (int)n;
+        total *= n;
+    }
+    const long long cap = std::max<long long>(64, 2 * (long long)N);
+    while (total > cap) {
+        int dm = 0;
+        for (int d = 1; d < g.ndim; ++d)
+            if (la.grid.nc[d] > la.grid.nc[dm]) dm = d;
+        int n = la.grid.nc[dm] / 2;
+        if (n < 3) n = 1;
+        total = total / la.grid.nc[dm] * n;
+        la.grid.nc[dm] = n;
+    }
+    for (int d = 0; d < g.ndim; ++d) la.grid.cell_scale[d] = la.grid.nc[d] / g.box[d];
+    la.grid.total = total;
+    r.cell_of.reset(new DBuf<uint32_t>(N));
+    r.slot.reset(new DBuf<uint32_t>(N));
+    r.items.reset(new DBuf<uint32_t>(N));
+    r.row_count.reset(new DBuf<uint32_t>(N + 1));
+    r.row_start.reset(new DBuf<uint32_t>(N + 1));
+    r.cell_count.reset(new DBuf<uint32_t>((size_t)total + 1));
+    r.cell_start.reset(new DBuf<uint32_t>((size_t)total + 1));
+    const size_t tb = rcp::scan_tmp_bytes(std::max<size_t>((size_t)total, N) + 8);
+    r.scan_tmp.reset(new DBuf<char>(tb));
+    la.cell_of = r.cell_of->p;
+    la.slot = r.slot->p;
+    la.items = r.items->p;
+    la.row_count = r.row_count->p;
+    la.row_start = r.row_start->p;
+    la.cell_count = r.cell_count->p;
+    la.cell_start = r.cell_start->p;
+    if (rcp::launch_cells(la, r.scan_tmp->p, tb, s)) throw rcp::HipFailure("cells launch");
+    if (rcp::launch_rows_count(la, s)) throw rcp::HipFailure("rows count launch");
+    if (rcp::launch_scan_u32(r.row_count->p, r.row_start->p, N, r.scan_tmp->p, s))
+        throw rcp::HipFailure("scan launch");
+    rcp::hip_check(hipMemcpyAsync(&r.nnz, r.row_start->p + N, sizeof(uint32_t),
+                                  hipMemcpyDeviceToHost, s), "nnz");
+    rcp::hip_check(hipStreamSynchronize(s), "sync");
+    r.nbr.reset(new DBuf<uint32_t>((size_t)r.nnz + 1));
+    la.nbr = r.nbr->p;
+    la.nbr_cap = (size_t)r.nnz + 1;
+    if (rcp::launch_rows_fill(la, s)) throw rcp::HipFailure("rows fill launch");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rcp_last_error(void) { return rcp::g_last_error.c_str(); }
+
+int rcp_device_info(char* buf, size_t buflen)
+{
+    return guarded([&]() {
+        int dev = 0;
+        rcp::hip_check(hipGetDevice(&dev), "hipGetDevice");
+        hipDeviceProp_t p;
+        rcp::hip_check(hipGetDeviceProperties(&p, dev), "hipGetDeviceProperties");
+        std::snprintf(buf, buflen, "%s (%s) CUs=%d HBM=%.1f GiB", p.name, p.gcnArchName,
+                      p.multiProcessorCount, p.totalGlobalMem / 1073741824.0);
+        return RCP_OK;
+    });
+}
+
+void rcp_run_config_init(rcp_run_config* c)
+{
+    std::memset(c, 0, sizeof(*c));
+    c->reorder_interval = 2000;
+    c->initial_phi_target = 0.55;
+}
+
+int rcp_engine_create(const rcp_run_config* cfg, rcp_engine** out)
+{
+    *out = nullptr;
+    return guarded([&]() {
+        if (!cfg) throw rcp::ArgError("rcp: null config");
+        std::unique_ptr<rcp_engine> h(new rcp_engine);
+        h->e.reset(new rcp::Engine(*cfg));
+        *out = h.release();
+        return RCP_OK;
+    });
+}
+
+int rcp_engine_run(rcp_engine* e, size_t nsteps, size_t progress_interval, int capture_positions,
+                   size_t trajectory_interval, rcp_observer_fn fn, void* user)
+{
+    return guarded([&]() {
+        if (!e) throw rcp::ArgError("rcp: null engine");
+        return e->e->run(nsteps, progress_interval, capture_positions != 0, trajectory_interval,
+                         fn, user);
+    });
+}
+
+int rcp_engine_finish(rcp_engine* e, int capture_positions, size_t trajectory_interval,
+                      size_t progress_interval, rcp_observer_fn fn, void* user)
+{
+    return guarded([&]() {
+        if (!e) throw rcp::ArgError("rcp: null engine");
+        return e->e->finish(capture_positions != 0, trajectory_interval, progress_interval, fn,
+                            user);
+    });
+}
+
+int rcp_engine_sync(rcp_engine* e)
+{
+    return guarded([&]() {
+        if (!e) throw rcp::ArgError("rcp: null engine");
+        rcp::hip_check(hipStreamSynchronize(e->e->stream()), "sync");
+        return RCP_OK;
+    });
+}
+
+int rcp_engine_info(const rcp_engine* e, rcp_result_info* info)
+{
+    return guarded([&]() {
+        if (!e) throw rcp::ArgError("rcp: null engine");
+        info->N = e->e->N();
+        info->ndim = e->e->ndim();
+        info->steps = e->e->steps();
+        info->phi = e->e->phi();
+        info->max_min_dist = e->e->max_min_dist();
+        info->force_magnitude = e->e->force_magnitude();
+        info->done = e->e->done() ? 1 : 0;
+        info->trace_len = e->e->trace().size();
+        return RCP_OK;
+    });
+}
+
+int rcp_engine_result(rcp_engine* e, double* positions, double* diameters,
+                      uint64_t* particle_origin, double* box, int8_t* walls,
+                      double* phi_history, double* force_history, double* energy_history,
+                      double* max_overlap_history, double* mu_history, double* alpha_history,
+                      int8_t* mu_flag_history)
+{
+    return guarded([&]() {
+        if (!e) throw rcp::ArgError("rcp: null engine");
+        rcp::Engine& E = *e->e;
+        if (positions) {
+            std::vector<double> x;
+            E.current_positions(x);
+            std::memcpy(positions, x.data(), x.size() * sizeof(double));
+        }
+        if (diameters) {
+            std::vector<double> D;
+            E.current_diameters(D);
+            std::memcpy(diameters, D.data(), D.size() * sizeof(double));
+        }
+        if (particle_origin) {
+            std::vector<uint64_t> o;
+            E.origin(o);
+            std::memcpy(particle_origin, o.data(), o.size() * sizeof(uint64_t));
+        }
+        if (box) std::memcpy(box, E.box().data(), E.ndim() * sizeof(double));
+        if (walls) std::memcpy(walls, E.walls().data(), E.ndim());
+        E.histories(phi_history, force_history, energy_history, max_overlap_history,
+                    mu_history, alpha_history, mu_flag_history);
+        return RCP_OK;
+    });
+}
+
+int rcp_engine_trace(const rcp_engine* e, size_t k, size_t* step, double* phi, double* force,
+                     double* energy, double* max_min_dist, double* positions,
+                     double* diameters, int* has_positions)
+{
+    return guarded([&]() {
+        if (!e) throw rcp::ArgError("rcp: null engine");
+        const auto& tr = e->e->trace();
+        if (k >= tr.size()) throw rcp::ArgError("rcp: trace index out of range");
+        const rcp::TraceSample& t = tr[k];
+        if (step) *step = t.step;
+        if (phi) *phi = t.phi;
+        if (force) *force = t.force;
+        if (energy) *energy = t.energy;
+        if (max_min_dist) *max_min_dist = t.max_min_dist;
+        if (positions && t.has_positions)
+            std::memcpy(positions, t.positions.data(), t.positions.size() * sizeof(double));
+        if (diameters && !t.diameters.empty())
+            std::memcpy(diameters, t.diameters.data(), t.diameters.size() * sizeof(double));
+        if (has_positions) *has_positions = t.has_positions ? 1 : 0;
+        return RCP_OK;
+    });
+}
+
+int rcp_engine_stats(const rcp_engine* e, uint64_t* list_builds, uint64_t* list_entries,
+                     uint64_t* reorders)
+{
+    return guarded([&]() {
+        if (!e) throw rcp::ArgError("rcp: null engine");
+        if (list_builds) *list_builds = e->e->list_builds();
+        if (list_entries) *list_entries = e->e->list_entries();
+        if (reorders) *reorders = e->e->reorders();
+        return RCP_OK;
+    });
+}
+
+void rcp_engine_destroy(rcp_engine* e)
+{
+    try {
+        delete e;
+    } catch (...) {
+    }
+}
+
+int rcp_run_packing_observed(const rcp_run_config* cfg, size_t progress_interval,
+                             int capture_positions, size_t trajectory_interval,
+                             rcp_observer_fn fn, void* user, rcp_engine** out)
+{
+    *out = nullptr;
+    rcp_engine* e = nullptr;
+    int rc = rcp_engine_create(cfg, &e);
+    if (rc) return rc;
+    rc = rcp_engine_run(e, 0, progress_interval, capture_positions, trajectory_interval, fn, user);
+    if (rc == RCP_OK)
+        rc = rcp_engine_finish(e, capture_positions, trajectory_interval, progress_interval, fn,
+                               user);
+    if (rc) {
+        std::string msg = rcp::g_last_error;
+        rcp_engine_destroy(e);
+        rcp::g_last_error = msg;
+        return rc;
+    }
+    *out = e;
+    return RCP_OK;
+}
+
+// --------------------------------------------------------- kernel level
+
+int rcp_forces(size_t N, size_t ndim, const double* x, const double* D, const double* box,
+               const int8_t* walls, double mu, const double* rk, double sumD_active, int exact,
+               double* F, double* scalars)
+{
+    return guarded([&]() {
+        check_dims(N, ndim);
+        rcp::Geometry g = make_geometry(N, ndim, box, walls);
+        Stream st;
+        DBuf<double> dx(N * ndim), dD(N), dF(N * ndim), dmag(N), dpart(((N + 255) / 256 + 1) * rcp::S_NUM),
+            dres(rcp::S_NUM);
+        DBuf<unsigned int> dcnt(1);
+        dx.up(x, N * ndim, st.s);
+        dD.up(D, N, st.s);
+        rcp::hip_check(hipMemsetAsync(dcnt.p, 0, sizeof(unsigned int), st.s), "memset");
+        std::unique_ptr<DBuf<double>> drk;
+        if (rk) {
+            drk.reset(new DBuf<double>(N));
+            drk->up(rk, N, st.s);
+        }
+        const double Dmax = *std::max_element(D, D + N);
+        const double Dmin = *std::min_element(D, D + N);
+        // contacts need d < (Di+Dj)/2; a small positive shell keeps every
+        // contact in the list (the kernel re-tests d2 < r^2 exactly)
+        DeviceRows rows;
+        device_rows(g, dx.p, dD.p, Dmax, std::max(0.05 * Dmin, 1e-12), 0, rows, st.s);
+        std::unique_ptr<DBuf<double>> dcontrib;
+        if (exact) dcontrib.reset(new DBuf<double>(((size_t)rows.nnz + 1) * 4));
+        rcp::StepArgs a;
+        a.g = g;
+        a.x_in = dx.p;
+        a.D0 = dD.p;
+        a.row_start = rows.row_start->p;
+        a.nbr = rows.nbr->p;
+        a.kappa = 1.0;
+        a.use_rk = rk ? 1 : 0;
+        a.rk = rk ? drk->p : nullptr;
+        a.do_adam = 0;
+        a.F_out = dF.p;
+        a.mag_out = dmag.p;
+        a.contrib = exact ? dcontrib->p : nullptr;
+        a.partials = dpart.p;
+        a.counter = dcnt.p;
+        a.result = dres.p;
+        if (rcp::launch_step(a, st.s)) throw rcp::HipFailure("step launch");
+        if (exact && rcp::launch_exact_reduce(a, st.s)) throw rcp::HipFailure("exact launch");
+        double r[rcp::S_NUM];
+        dres.down(r, rcp::S_NUM, st.s);
+        dF.down(F, N * ndim, st.s);
+        st.sync();
+        const double count = 1.0 + r[rcp::S_CNT];
+        scalars[0] = std::pow(r[rcp::S_U] / count, 2);
+        scalars[1] = r[rcp::S_MAXOV];
+        scalars[2] = r[rcp::S_LC] / count;
+        scalars[3] = -r[rcp::S_FM] / count;
+        scalars[4] = r[rcp::S_DK] + mu * (sumD_active >= 0.0 ? sumD_active : r[rcp::S_SUMD]);
+        scalars[5] = count;
+        return RCP_OK;
+    });
+}
+
+int rcp_pairs(size_t N, size_t ndim, const double* x, const double* D, const double* box,
+              uint64_t* row_off, uint32_t* nbr, size_t nbr_capacity, size_t* npairs)
+{
+    return guarded([&]() {
+        check_dims(N, ndim);
+        rcp::Geometry g = make_geometry(N, ndim, box, nullptr);
+        Stream st;
+        DBuf<double> dx(N * ndim), dD(N);
+        dx.up(x, N * ndim, st.s);
+        dD.up(D, N, st.s);
+        std::vector<double> Dv(D, D + N);
+        const double t_shell = rcp::reference_shell(Dv);
+        const double Dmax = *std::max_element(D, D + N);
+        DeviceRows rows;
+        device_rows(g, dx.p, dD.p, Dmax, t_shell, 1, rows, st.s);
+        *npairs = rows.nnz;
+        std::vector<uint32_t> rs(N + 1);
+        rows.row_start->down(rs.data(), N + 1, st.s);
+        if (nbr && nbr_capacity >= rows.nnz) rows.nbr->down(nbr, rows.nnz, st.s);
+        st.sync();
+        for (size_t i = 0; i <= N; ++i) row_off[i] = rs[i];
+        if (nbr && nbr_capacity < rows.nnz) throw rcp::ArgError("rcp_pairs: nbr capacity too small");
+        return RCP_OK;
+    });
+}
+
+int rcp_mean_force(size_t N, size_t ndim, const double* F, double* out)
+{
+    return guarded([&]() {
+        check_dims(N, ndim);
+        Stream st;
+        rcp::Geometry g;
+        g.N = (int)N;
+        g.ndim = (int)ndim;
+        DBuf<double> dF(N * ndim), dmag(N), dpart(((N + 255) / 256 + 1) * rcp::S_NUM),
+            dres(rcp::S_NUM);
+        DBuf<unsigned int> dcnt(1);
+        dF.up(F, N * ndim, st.s);
+        rcp::hip_check(hipMemsetAsync(dcnt.p, 0, sizeof(unsigned int), st.s), "memset");
+        rcp::StepArgs a;
+        a.g = g;
+        a.mag_out = dmag.p;
+        a.partials = dpart.p;
+        a.counter = dcnt.p;
+        a.result = dres.p;
+        if (rcp::launch_mean_force(a, dF.p, 1, st.s)) throw rcp::HipFailure("mean force launch");
+        double r[rcp::S_NUM];
+        dres.down(r, rcp::S_NUM, st.s);
+        st.sync();
+        const double mean_mag = r[rcp::S_SMAG] / double((uint32_t)r[rcp::S_MCNT]);
+        *out = mean_mag / std::sqrt(double(ndim));
+        return RCP_OK;
+    });
+}
+
+int rcp_adam_positions(size_t N, size_t ndim, const double* F, double* m, double* v, double* x,
+                       const double* box, double alpha, double inv_b1, double inv_b2)
+{
+    return guarded([&]() {
+        check_dims(N, ndim);
+        rcp::Geometry g = make_geometry(N, ndim, box, nullptr);
+        Stream st;
+        DBuf<double> dF(N * ndim), dm(N * ndim), dv(N * ndim), dx(N * ndim), dxo(N * ndim);
+        dF.up(F, N * ndim, st.s);
+        dm.up(m, N * ndim, st.s);
+        dv.up(v, N * ndim, st.s);
+        dx.up(x, N * ndim, st.s);
+        if (rcp::launch_adam_pass(g, dF.p, dm.p, dv.p, dx.p, dxo.p, alpha, inv_b1, inv_b2, st.s))
+            throw rcp::HipFailure("adam launch");
+        dm.down(m, N * ndim, st.s);
+        dv.down(v, N * ndim, st.s);
+        dxo.down(x, N * ndim, st.s);
+        st.sync();
+        return RCP_OK;
+    });
+}
+
+int rcp_reorder_perm(size_t N, size_t ndim, const double* x, const double* box, uint32_t* perm)
+{
+    return guarded([&]() {
+        check_dims(N, ndim);
+        Stream st;
+        DBuf<double> dx(N * ndim), dbox(ndim);
+        dx.up(x, N * ndim, st.s);
+        DBuf<unsigned long long> k(N), k2(N);
+        DBuf<uint32_t> v(N), v2(N), hist(rcp::sort_hist_entries(N));
+        DBuf<char> tmp(rcp::scan_tmp_bytes(rcp::sort_hist_entries(N) + 8));
+        if (rcp::launch_reorder_keys(dx.p, (int)N, (int)ndim, box, k.p, v.p, st.s))
+            throw rcp::HipFailure("keys launch");
+        rcp::SortTmp t;
+        t.keys_alt = k2.p;
+        t.vals_alt = v2.p;
+        t.hist = hist.p;
+        t.scan_tmp = tmp.p;
+        if (rcp::launch_radix_sort(k.p, v.p, N, 64, t, st.s)) throw rcp::HipFailure("sort launch");
+        v.down(perm, N, st.s);
+        st.sync();
+        return RCP_OK;
+    });
+}
+
+int rcp_scale_diameters(size_t N, size_t ndim, const double* D, double phi_target,
+                        const double* box, int fix_height, double* out, double* factor)
+{
+    return guarded([&]() {
+        check_dims(N, ndim);
+        std::vector<double> Dv(D, D + N), bv(box, box + ndim), o;
+        const double f = rcp::scale_diameters(Dv, phi_target, bv, ndim, fix_height != 0, o);
+        std::memcpy(out, o.data(), N * sizeof(double));
+        if (factor) *factor = f;
+        return RCP_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,169 @@
+// Single-packing engine: device-resident state + host schedule controller.
+#pragma once
+
+#include "rcp_amd.h"
+#include "rcp_kernels.h"
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+namespace rcp {
+
+extern thread_local std::string g_last_error;
+
+struct ArgError : std::invalid_argument {
+    using std::invalid_argument::invalid_argument;
+};
+struct RtError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct HipFailure : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char* what);
+void* dev_alloc(size_t bytes);
+void dev_free(void* p);
+double sphere_volume(double r, std::size_t ndim);
+double scale_diameters(const std::vector<double>& D, double phi_target,
+                       const std::vector<double>& box, std::size_t ndim, bool fix_height,
+                       std::vector<double>& out);
+double reference_shell(std::vector<double> D);
+
+struct TraceSample {
+    std::size_t step = 0;
+    double phi = 0, force = 0, energy = 0, max_min_dist = 0;
+    std::vector<double> positions, diameters;
+    bool has_positions = false;
+};
+
+class Engine {
+public:
+    explicit Engine(const rcp_run_config& c);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    int run(std::size_t nsteps, std::size_t progress_interval, bool capture,
+            std::size_t trajectory_interval, rcp_observer_fn fn, void* user);
+    int finish(bool capture, std::size_t trajectory_interval, std::size_t progress_interval,
+               rcp_observer_fn fn, void* user);
+
+    std::size_t N() const { return N_; }
+    std::size_t ndim() const { return nd_; }
+    std::size_t steps() const { return hist_filled_; }
+    bool done() const { return done_; }
+    double phi() const { return phi_; }
+    double max_min_dist() const { return max_min_dist_; }
+    double force_magnitude() const { return force_magnitude_; }
+    const std::vector<double>& box() const { return box_; }
+    const std::vector<int8_t>& walls() const { return walls_; }
+    const std::vector<TraceSample>& trace() const { return trace_; }
+    hipStream_t stream() const { return stream_; }
+
+    void current_positions(std::vector<double>& x);
+    void current_diameters(std::vector<double>& D);
+    void origin(std::vector<uint64_t>& o);
+    void histories(double* phi, double* F, double* U, double* ov, double* mu, double* al,
+                   int8_t* fl) const;
+    uint64_t list_builds() const { return list_builds_; }
+    uint64_t list_entries() const { return list_entries_; }
+    uint64_t reorders() const { return reorders_; }
+
+private:
+    struct Scalars { double v[S_NUM]; };
+    enum class Osc { Inactive, Active, Final, Done };
+
+    template <class T> T* dalloc(size_t n);
+    void dfree(void* p);
+    void alloc_device();
+    void ensure_scan_tmp(size_t n);
+    void recompute_d0_cache(const std::vector<double>& D0, const std::vector<int8_t>& fixed);
+    void download_d0(std::vector<double>& D0, std::vector<int8_t>& fixed);
+    void reorder();
+    void build_lists();
+    void configure_grid(double reach, CellGrid& grid);
+    void build_rows(ListArgs& la);
+    Scalars step_device(bool do_adam, double alpha, double inv_b1, double inv_b2, int use_rk,
+                        const double* rk = nullptr);
+    void finalize_scalars(const Scalars& s, double mu_eff, bool use_active);
+    bool schedule_top(std::size_t step);
+    void schedule_alpha(std::size_t step);
+    double current_volume_now();
+    void lock_target_phi(double box_volume);
+    void release_cascade();
+    bool step_once();
+    int record(std::size_t step, bool capture, bool final_sample, rcp_observer_fn fn,
+               void* user);
+
+    // configuration
+    std::size_t N_ = 0, nd_ = 0;
+    rcp_run_config opt_{};
+    std::vector<double> box_;
+    std::vector<int8_t> walls_;
+    int curved_T_ = 0;
+    double phi_modifier_ = 1.0;
+    bool has_fixed_ = false, fixed_nonempty_ = false, cascade_released_ = false;
+
+    // schedule state (RG:4655-4800 names)
+    std::size_t step_cap_ = 0, rung1_deadline_ = 0;
+    double phi_ = 0.0, t_ = 0.0;
+    double alpha_ = 0.0, alpha_max_ = 0.0, mu_ = 0.0;
+    int mu_flag_ = 1;
+    double mu_change_ = 0.;
+    bool runtime_fix_ = false, target_locked_ = false;
+    double kappa_ = 1.0, kappa_D_ = 1.0, dkappa_ = 1.0, m_kappa_ = 0.0, v_kappa_ = 0.0;
+    std::size_t last_alpha_update_ = 0;
+    bool mu_decay_active_ = false;
+    double mu_decay_target_ = 0.0;
+    std::size_t mu_decay_next_tick_ = 0;
+    Osc osc_ = Osc::Inactive;
+    double osc_mu_0_ = 0.0, osc_mu_m1_ = 0.0;
+    std::size_t osc_start_ = 0, osc_period_ = 0, osc_t_relax_ = 0, osc_final_start_ = 0,
+                osc_consec_ = 0;
+    double D0_min_ = 0.0, S_D0_ = 0.0, D0max_free_ = 0.0, D0max_fixed_ = 0.0, shell0_ = 0.0;
+    // force-evaluation outputs
+    double U_ = 0.0, Lc_ = 0.0, Fmean_ = 0.0, max_min_dist_ = 0.0, mean_force_ = 0.0;
+    double force_magnitude_ = 0.0, maxdisp2_ = 0.0;
+    // histories
+    std::vector<double> phi_h_, F_h_, U_h_, ov_h_, mu_h_, al_h_;
+    std::vector<int8_t> fl_h_;
+    std::size_t hist_filled_ = 0, steps_var_ = 0;
+    bool done_ = false, initial_recorded_ = false;
+    std::vector<TraceSample> trace_;
+
+    // neighbour lists
+    double t_list_ = 0.0, kappa_b_ = 1.0, fh_drift_ = 0.0;
+    bool need_build_ = true;
+    uint64_t list_builds_ = 0, list_entries_ = 0, reorders_ = 0;
+    size_t nnz_ = 0;
+
+    // device
+    Geometry geo_;
+    hipStream_t stream_ = nullptr;
+    std::vector<void*> allocs_;
+    double* x_[2] = {nullptr, nullptr};
+    int cur_ = 0;
+    double *m_ = nullptr, *v_ = nullptr, *tmp_f64_ = nullptr, *x_anchor_ = nullptr,
+           *D0_ = nullptr, *F_ = nullptr, *mag_ = nullptr, *contrib_ = nullptr,
+           *partials_ = nullptr, *result_ = nullptr, *rk_tmp_ = nullptr;
+    double* h_result_ = nullptr;
+    int8_t *fixed_ = nullptr, *fixed_tmp_ = nullptr;
+    unsigned long long *origin_ = nullptr, *origin_tmp_ = nullptr, *keys_ = nullptr,
+                       *keys_alt_ = nullptr;
+    uint32_t *cell_of_ = nullptr, *slot_ = nullptr, *items_ = nullptr, *row_count_ = nullptr,
+             *row_start_ = nullptr, *nbr_ = nullptr, *cell_count_ = nullptr,
+             *cell_start_ = nullptr, *vals_ = nullptr, *vals_alt_ = nullptr, *hist_ = nullptr;
+    unsigned int* counter_ = nullptr;
+    char* scan_tmp_ = nullptr;
+    size_t scan_tmp_bytes_ = 0, cell_cap_ = 0, nbr_cap_ = 0;
+};
+
+}  // namespace rcp

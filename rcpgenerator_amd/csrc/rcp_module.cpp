@@ -1,0 +1,479 @@
+// pybind11 module `_rcpgenerator`: the reference binding's five functions
+// (KD-physics/RCPGenerator python_code/python/src/bindings/module.cpp,
+// MOD:200-297) with byte-identical dict keys and value types (lists, not
+// arrays), implemented over the C ABI of include/rcp_amd.h.  Adds an
+// `Engine` class (step-wise runs for benchmarks and batching).
+#include "rcp_amd.h"
+
+#include <Python.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <cstdint>
+#include <cstdlib>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace py = pybind11;
+
+namespace {
+
+int g_threads = 1;
+
+[[noreturn]] void raise_rc(int rc)
+{
+    const std::string msg = rcp_last_error();
+    if (rc == RCP_E_ARG) throw py::value_error(msg);
+    if (rc == RCP_E_NOMEM) throw std::bad_alloc();
+    throw std::runtime_error(msg);
+}
+
+template <typename T>
+void take(const py::dict& d, const char* key, T& out)
+{
+    if (d.contains(py::str(key))) out = d[py::str(key)].cast<T>();
+}
+
+std::vector<int8_t> cast_walls(const py::handle& h)
+{
+    std::vector<int8_t> w;
+    for (const auto item : h) w.push_back(py::cast<int8_t>(item));
+    return w;
+}
+
+// Fast list builders (PyFloat/PyLong objects straight into new lists).
+py::list list_f64(const double* p, size_t n)
+{
+    PyObject* l = PyList_New((Py_ssize_t)n);
+    if (!l) throw py::error_already_set();
+    for (size_t i = 0; i < n; ++i) PyList_SET_ITEM(l, (Py_ssize_t)i, PyFloat_FromDouble(p[i]));
+    return py::reinterpret_steal<py::list>(l);
+}
+py::list list_nested(const double* p, size_t n, size_t w)
+{
+    PyObject* l = PyList_New((Py_ssize_t)n);
+    if (!l) throw py::error_already_set();
+    for (size_t i = 0; i < n; ++i) {
+        PyObject* r = PyList_New((Py_ssize_t)w);
+        for (size_t d = 0; d < w; ++d) PyList_SET_ITEM(r, (Py_ssize_t)d, PyFloat_FromDouble(p[i * w + d]));
+        PyList_SET_ITEM(l, (Py_ssize_t)i, r);
+    }
+    return py::reinterpret_steal<py::list>(l);
+}
+template <class T>
+py::list list_int(const T* p, size_t n)
+{
+    PyObject* l = PyList_New((Py_ssize_t)n);
+    if (!l) throw py::error_already_set();
+    for (size_t i = 0; i < n; ++i)
+        PyList_SET_ITEM(l, (Py_ssize_t)i, PyLong_FromLongLong((long long)p[i]));
+    return py::reinterpret_steal<py::list>(l);
+}
+
+// Flatten a list of coordinate rows (any sequence of sequences of floats).
+void flatten_positions(const py::handle& h, std::vector<double>& flat, size_t& N, size_t& nd)
+{
+    py::sequence rows = py::reinterpret_borrow<py::sequence>(h);
+    N = (size_t)py::len(rows);
+    nd = 0;
+    flat.clear();
+    for (size_t i = 0; i < N; ++i) {
+        py::sequence r = rows[i];
+        const size_t w = (size_t)py::len(r);
+        if (i == 0) {
+            nd = w;
+            flat.reserve(N * nd);
+        } else if (w != nd) {
+            throw py::value_error("positions must all have the same length");
+        }
+        for (size_t d = 0; d < w; ++d) flat.push_back(r[d].cast<double>());
+    }
+}
+
+struct RunInputs {
+    std::vector<double> x, D, box;
+    std::vector<int8_t> fixed, walls;
+    rcp_run_config cfg;
+};
+
+void parse_run(const py::dict& input, const py::dict& config, const py::dict& options, RunInputs& r)
+{
+    rcp_run_config_init(&r.cfg);
+    size_t N = 0, nd = 0;
+    if (input.contains("positions")) flatten_positions(input["positions"], r.x, N, nd);
+    take(input, "diameters", r.D);
+    take(input, "fixed", r.fixed);
+    take(config, "box", r.box);
+    if (config.contains("walls")) r.walls = cast_walls(config["walls"]);
+    uint32_t neighbor_max = 0, seed = 0;
+    bool fix_height = false;
+    take(config, "neighbor_max", neighbor_max);
+    take(config, "seed", seed);
+    take(config, "fix_height", fix_height);
+    if (r.D.size() != N) throw py::value_error("diameters must have one entry per position");
+    // the reference resets a wrongly sized mask to all zeros (RG:4368)
+    if (!r.fixed.empty() && r.fixed.size() != N) r.fixed.assign(N, 0);
+    rcp_run_config& c = r.cfg;
+    c.N = N;
+    c.ndim = nd;
+    c.positions = r.x.data();
+    c.diameters = r.D.data();
+    c.fixed = r.fixed.empty() ? nullptr : r.fixed.data();
+    c.box = r.box.data();
+    c.nbox = r.box.size();
+    c.walls = r.walls.data();
+    c.nwalls = r.walls.size();
+    c.neighbor_max = neighbor_max;
+    c.seed = seed;
+    c.fix_height = fix_height;
+    size_t max_steps = 0, reorder_interval = 2000;
+    bool fix_diameter = false;
+    take(options, "max_steps", max_steps);
+    take(options, "fix_diameter", fix_diameter);
+    take(options, "reorder_interval", reorder_interval);
+    c.max_steps = max_steps;
+    c.fix_diameter = fix_diameter;
+    c.reorder_interval = reorder_interval;
+    if (options.contains("mu")) { c.has_mu = 1; c.mu = options["mu"].cast<double>(); }
+    if (options.contains("target_phi")) {
+        c.has_target_phi = 1;
+        c.target_phi = options["target_phi"].cast<double>();
+    }
+    if (options.contains("initial_phi_target")) {
+        c.has_initial_phi_target = 1;
+        c.initial_phi_target = options["initial_phi_target"].cast<double>();
+    }
+    bool exact = false;
+    take(options, "exact_reductions", exact);
+    c.exact_reductions = exact;
+}
+
+py::dict result_dict(rcp_engine* e)
+{
+    rcp_result_info info;
+    int rc = rcp_engine_info(e, &info);
+    if (rc) raise_rc(rc);
+    const size_t N = info.N, nd = info.ndim, s = info.steps;
+    std::vector<double> x(N * nd), D(N), box(nd), ph(s), F(s), U(s), ov(s), mu(s), al(s);
+    std::vector<uint64_t> origin(N);
+    std::vector<int8_t> walls(nd), fl(s);
+    rc = rcp_engine_result(e, x.data(), D.data(), origin.data(), box.data(), walls.data(),
+                           ph.data(), F.data(), U.data(), ov.data(), mu.data(), al.data(), fl.data());
+    if (rc) raise_rc(rc);
+    py::dict out;
+    out["positions"] = list_nested(x.data(), N, nd);
+    out["diameters"] = list_f64(D.data(), N);
+    out["box"] = list_f64(box.data(), nd);
+    out["walls"] = list_int(walls.data(), nd);
+    out["phi_history"] = list_f64(ph.data(), s);
+    out["force_history"] = list_f64(F.data(), s);
+    out["energy_history"] = list_f64(U.data(), s);
+    out["max_overlap_history"] = list_f64(ov.data(), s);
+    out["mu_history"] = list_f64(mu.data(), s);
+    out["alpha_history"] = list_f64(al.data(), s);
+    out["mu_flag_history"] = list_int(fl.data(), s);
+    // research instruments the reference leaves empty by default (MOD:169-177)
+    out["pp_F_sq_sum"] = py::list();
+    out["pp_F_sign_flips"] = py::list();
+    out["pp_pos_range_max"] = py::list();
+    out["pp_contact_sum"] = py::list();
+    out["pp_signflip_resets"] = py::list();
+    out["pp_window_steps"] = (size_t)0;
+    out["spike_log"] = py::list();
+    out["reorder_log"] = py::list();
+    out["steps"] = s;
+    out["phi"] = info.phi;
+    out["max_min_dist"] = info.max_min_dist;
+    out["force_magnitude"] = info.force_magnitude;
+    out["particle_origin"] = list_int(origin.data(), N);
+    return out;
+}
+
+py::dict trace_dict(rcp_engine* e)
+{
+    rcp_result_info info;
+    int rc = rcp_engine_info(e, &info);
+    if (rc) raise_rc(rc);
+    const size_t N = info.N, nd = info.ndim;
+    py::list steps, positions, diameters, phi, force, energy, mmd;
+    std::vector<double> x(N * nd), D(N);
+    for (size_t k = 0; k < info.trace_len; ++k) {
+        size_t st = 0;
+        double p = 0, f = 0, u = 0, m = 0;
+        int hp = 0;
+        rc = rcp_engine_trace(e, k, &st, &p, &f, &u, &m, x.data(), D.data(), &hp);
+        if (rc) raise_rc(rc);
+        steps.append(st);
+        phi.append(p);
+        force.append(f);
+        energy.append(u);
+        mmd.append(m);
+        diameters.append(list_f64(D.data(), N));
+        if (hp) positions.append(list_nested(x.data(), N, nd));
+    }
+    py::dict out;
+    out["steps"] = steps;
+    out["positions"] = positions;
+    out["diameters"] = diameters;
+    out["phi"] = phi;
+    out["force"] = force;
+    out["energy"] = energy;
+    out["max_min_dist"] = mmd;
+    return out;
+}
+
+struct Observer {
+    py::function fn;
+    size_t N = 0, nd = 0;
+    bool failed = false;
+    std::unique_ptr<py::error_already_set> err;
+};
+
+int observer_thunk(void* user, const rcp_observation* o, const double* positions)
+{
+    Observer* ob = static_cast<Observer*>(user);
+    try {
+        py::dict update;
+        update["step"] = o->step;
+        update["phi"] = o->phi;
+        update["force_magnitude"] = o->force_magnitude;
+        update["energy"] = o->energy;
+        update["max_min_dist"] = o->max_min_dist;
+        if (positions) update["positions"] = list_nested(positions, ob->N, ob->nd);
+        ob->fn(update);
+        return 0;
+    } catch (py::error_already_set& e) {
+        ob->failed = true;
+        ob->err.reset(new py::error_already_set(std::move(e)));
+        return 1;
+    }
+}
+
+struct EngineHandle {
+    rcp_engine* e = nullptr;
+    RunInputs in;
+    ~EngineHandle() { if (e) rcp_engine_destroy(e); }
+};
+
+py::dict run_observed_impl(const py::dict& input, const py::dict& config, size_t progress_interval,
+                           bool capture_positions, size_t trajectory_interval, py::object callback,
+                           const py::dict& options, bool with_trace)
+{
+    EngineHandle h;
+    parse_run(input, config, options, h.in);
+    Observer ob;
+    rcp_observer_fn fn = nullptr;
+    if (!callback.is_none()) {
+        ob.fn = callback.cast<py::function>();
+        ob.N = h.in.cfg.N;
+        ob.nd = h.in.cfg.ndim;
+        fn = observer_thunk;
+    }
+    int rc;
+    if (fn) {
+        rc = rcp_run_packing_observed(&h.in.cfg, progress_interval, capture_positions,
+                                      trajectory_interval, fn, &ob, &h.e);
+    } else {
+        py::gil_scoped_release nogil;
+        rc = rcp_run_packing_observed(&h.in.cfg, progress_interval, capture_positions,
+                                      trajectory_interval, nullptr, nullptr, &h.e);
+    }
+    if (ob.failed && ob.err) throw *ob.err;
+    if (rc) raise_rc(rc);
+    py::dict res = result_dict(h.e);
+    if (!with_trace) return res;
+    py::dict out;
+    out["result"] = res;
+    out["trace"] = trace_dict(h.e);
+    return out;
+}
+
+// Step-wise engine for benchmarks and drivers.
+class PyEngine {
+public:
+    PyEngine(const py::dict& input, const py::dict& config, const py::dict& options)
+    {
+        parse_run(input, config, options, h_.in);
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = rcp_engine_create(&h_.in.cfg, &h_.e);
+        }
+        if (rc) raise_rc(rc);
+    }
+    size_t run(size_t nsteps)
+    {
+        rcp_result_info a, b;
+        rcp_engine_info(h_.e, &a);
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = rcp_engine_run(h_.e, nsteps, 0, 0, 0, nullptr, nullptr);
+        }
+        if (rc) raise_rc(rc);
+        rcp_engine_info(h_.e, &b);
+        return b.steps - a.steps;
+    }
+    void sync()
+    {
+        int rc;
+        {
+            py::gil_scoped_release nogil;
+            rc = rcp_engine_sync(h_.e);
+        }
+        if (rc) raise_rc(rc);
+    }
+    py::dict result() { return result_dict(h_.e); }
+    py::dict info()
+    {
+        rcp_result_info i;
+        int rc = rcp_engine_info(h_.e, &i);
+        if (rc) raise_rc(rc);
+        uint64_t builds = 0, entries = 0, reorders = 0;
+        rcp_engine_stats(h_.e, &builds, &entries, &reorders);
+        py::dict d;
+        d["N"] = i.N;
+        d["ndim"] = i.ndim;
+        d["steps"] = i.steps;
+        d["phi"] = i.phi;
+        d["max_min_dist"] = i.max_min_dist;
+        d["force_magnitude"] = i.force_magnitude;
+        d["done"] = (bool)i.done;
+        d["list_builds"] = builds;
+        d["list_entries"] = entries;
+        d["reorders"] = reorders;
+        return d;
+    }
+
+private:
+    EngineHandle h_;
+};
+
+}  // namespace
+
+PYBIND11_MODULE(_rcpgenerator, m)
+{
+    m.doc() = "MI355X (HIP, gfx950) engine behind rcpgenerator.Packing";
+    if (const char* s = std::getenv("OMP_NUM_THREADS")) {
+        const int v = std::atoi(s);
+        if (v >= 1) g_threads = v;
+    }
+
+    m.def(
+        "set_num_threads",
+        [](int n) {
+            if (n < 1) throw std::invalid_argument("num_threads must be >= 1");
+            g_threads = n;
+        },
+        py::arg("n"),
+        "Host thread count hint (kept for API compatibility; the GPU engine's host side "
+        "is single-threaded).");
+    m.def("get_num_threads", []() { return g_threads; });
+
+    m.def(
+        "initialize_particles",
+        [](const py::dict& cfg) {
+            rcp_init_config c;
+            rcp_init_config_init(&c);
+            std::vector<double> box, custom;
+            std::vector<int8_t> walls;
+            std::string type = "mono";
+            size_t N = 0, nd = 0;
+            uint64_t seed = 0;
+            bool fix_height = false;
+            take(cfg, "phi", c.phi);
+            take(cfg, "N", N);
+            take(cfg, "Ndim", nd);
+            take(cfg, "box", box);
+            take(cfg, "fix_height", fix_height);
+            take(cfg, "seed", seed);
+            if (cfg.contains("walls")) walls = cast_walls(cfg["walls"]);
+            if (cfg.contains("dist")) {
+                py::dict d = cfg["dist"].cast<py::dict>();
+                take(d, "type", type);
+                take(d, "d", c.d);
+                take(d, "mu", c.mu);
+                take(d, "sigma", c.sigma);
+                take(d, "mu1", c.mu1);
+                take(d, "sigma1", c.sigma1);
+                take(d, "mu2", c.mu2);
+                take(d, "sigma2", c.sigma2);
+                take(d, "p", c.p);
+                take(d, "d1", c.d1);
+                take(d, "d2", c.d2);
+                take(d, "d_min", c.d_min);
+                take(d, "d_max", c.d_max);
+                take(d, "exponent", c.exponent);
+                take(d, "scale", c.scale);
+                take(d, "shape", c.shape);
+                take(d, "custom", custom);
+            }
+            c.N = N;
+            c.ndim = nd;
+            c.box = box.data();
+            c.nbox = box.size();
+            c.walls = walls.data();
+            c.nwalls = walls.size();
+            c.fix_height = fix_height;
+            c.seed = seed;
+            c.dist_type = type.c_str();
+            c.custom = custom.empty() ? nullptr : custom.data();
+            c.ncustom = custom.size();
+            const size_t ndo = nd < 2 ? 2 : nd;
+            std::vector<double> x(N * ndo), D(N), bo(ndo);
+            std::vector<int8_t> wo(ndo);
+            double factor = 1.0, phim = 1.0;
+            int rc;
+            {
+                py::gil_scoped_release nogil;
+                rc = rcp_initialize_particles(&c, x.data(), D.data(), bo.data(), wo.data(), &factor,
+                                              &phim);
+            }
+            if (rc) raise_rc(rc);
+            py::dict out;
+            out["positions"] = list_nested(x.data(), N, nd);
+            out["diameters"] = list_f64(D.data(), N);
+            out["box"] = list_f64(bo.data(), nd);
+            out["walls"] = list_int(wo.data(), nd);
+            out["diameter_scale_factor"] = factor;
+            out["phi_modifier"] = phim;
+            return out;
+        },
+        py::arg("config"));
+
+    m.def(
+        "run_packing",
+        [](const py::dict& input, const py::dict& config) {
+            return run_observed_impl(input, config, 0, false, 0, py::none(), py::dict(), false);
+        },
+        py::arg("input"), py::arg("config"));
+
+    m.def(
+        "run_packing_observed",
+        [](const py::dict& input, const py::dict& config, size_t progress_interval,
+           bool capture_positions, size_t trajectory_interval, py::object callback,
+           py::dict options) {
+            return run_observed_impl(input, config, progress_interval, capture_positions,
+                                     trajectory_interval, callback, options, true);
+        },
+        py::arg("input"), py::arg("config"), py::arg("progress_interval") = 0,
+        py::arg("capture_positions") = false, py::arg("trajectory_interval") = 0,
+        py::arg("callback") = py::none(), py::arg("options") = py::dict());
+
+    m.def("device_info", []() {
+        char buf[256];
+        int rc = rcp_device_info(buf, sizeof(buf));
+        if (rc) raise_rc(rc);
+        return std::string(buf);
+    });
+
+    py::class_<PyEngine>(m, "Engine")
+        .def(py::init<const py::dict&, const py::dict&, const py::dict&>(), py::arg("input"),
+             py::arg("config"), py::arg("options") = py::dict())
+        .def("run", &PyEngine::run, py::arg("nsteps"))
+        .def("sync", &PyEngine::sync)
+        .def("result", &PyEngine::result)
+        .def("info", &PyEngine::info);
+}
