@@ -1,0 +1,181 @@
+#!/usr/bin/env python3
+"""Headline benchmark: particle-updates/s of one large packing on one MI355X.
+
+Workload (the single-GPU configuration the metric is quoted on): plain
+method, 2-D, N = 1,000,000 disks, periodic unit square, log-normal diameters
+(mean 1.0, std 0.2), initial phi 0.10, uniform random positions from seed
+123, FP64 state.  A "step" is one pass of the hot path -- neighbour-list
+maintenance, forces + walls + reductions, schedule controller, kappa and
+per-DOF ADAM -- and a particle-update is one particle advanced one step, so
+value = N * steps / wall-seconds of the timed region.
+
+The timed region runs steps 1..K of a packing created (initializer, upload,
+initial reorder, first list build) before the clock starts, with inputs
+resident in HBM.  W warm-up steps run first on a separate engine built from
+the same initial state.  The dominant kernel (the fused step kernel) is
+timed live with HIP events on the engine's own stream; its algorithmic HBM
+bytes are A(2) = 8*(6*2+1) = 104 B per particle (x read+write, ADAM m and v
+read+write, D0 read) times N per launch.
+
+The CPU baseline runs the reference engine (built from its sources into
+oracle/_ref; AVX-512 build when the host supports it) with all host cores on
+the first steps of the same packing.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-updates/s at N=1e6; jammed packings/hour (batched+surrogate vs plain)"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload(N: int):
+    mu = -0.5 * math.log(1.04)
+    sigma = math.sqrt(math.log(1.04))
+    init = {"phi": 0.10, "N": N, "Ndim": 2, "box": [1.0, 1.0], "walls": [0, 0], "seed": 123,
+            "dist": {"type": "lognormal", "mu": mu, "sigma": sigma}}
+    return init
+
+
+def has_avx512() -> bool:
+    try:
+        with open("/proc/cpuinfo") as f:
+            return " avx512f " in f.read()
+    except OSError:
+        return False
+
+
+def cpu_baseline(inp, cfg, N, budget_s: float):
+    """Reference engine on the host cores: per-step cost from two short runs
+    (their difference removes the set-up), on the first steps of the same
+    packing."""
+    from oracle import oracle as O
+    if not O.have_ref():
+        return None
+    native = has_avx512() and os.path.exists(
+        os.path.join(O.REF_DIR, "native", "_rcpgenerator" + O.EXT_SUFFIX))
+    mod = O.ref_module(native=native)
+    cores = os.cpu_count() or 1
+    mod.set_num_threads(cores)
+
+    def timed(k):
+        t0 = time.perf_counter()
+        mod.run_packing_observed(inp, cfg, options={"max_steps": k})
+        return time.perf_counter() - t0
+
+    t1 = timed(1)
+    t4 = timed(4)
+    per = max((t4 - t1) / 3.0, 1e-6)
+    k = int(max(4, min(200, budget_s / per)))
+    tk = timed(1 + k)
+    per = max((tk - t1) / k, 1e-9)
+    return {"value": N / per, "unit": "particle-updates/s", "cores": cores,
+            "kind": "reference",
+            "sample": f"steps 2..{1 + k} of the same packing (run_packing_observed max_steps "
+                      f"{1 + k} minus max_steps 1), {'AVX-512' if native else 'portable'} "
+                      f"build, set_num_threads({cores})"}
+
+
+def read_traffic():
+    """HBM bytes per launch of the fused step kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled)."""
+    p = os.path.join(ROOT, "profiles", "k_step_2d_1M_traffic.json")
+    try:
+        with open(p) as f:
+            return json.load(f).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this engine runs one packing per GPU and this build measures one "
+              "GPU only; run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import rcpgenerator_amd as rcp
+    N, K, W = args.n, args.steps, args.warmup
+    log(f"[bench] device: {rcp._rcpgenerator.device_info()}")
+    t0 = time.perf_counter()
+    init = rcp.initialize_particles(workload(N))
+    log(f"[bench] initializer {time.perf_counter() - t0:.1f}s")
+    inp = {"positions": init["positions"], "diameters": init["diameters"]}
+    cfg = {"box": init["box"], "walls": init["walls"]}
+    opts = {"max_steps": 1_000_000, "time_kernels": True}
+
+    if W > 0:
+        warm = rcp.Engine(inp, cfg, opts)
+        warm.run(W)
+        warm.sync()
+        del warm
+        log(f"[bench] warm-up {W} steps done")
+    eng = rcp.Engine(inp, cfg, opts)
+    eng.sync()
+    base = eng.info()
+    t0 = time.perf_counter()
+    done = eng.run(K)
+    eng.sync()
+    dt = time.perf_counter() - t0
+    info = eng.info()
+    log(f"[bench] {done} steps in {dt:.3f}s; list builds {info['list_builds']}, "
+        f"reorders {info['reorders']}")
+    value = N * done / dt
+    launches = info["step_kernel_launches"] - base["step_kernel_launches"]
+    kms = (info["step_kernel_ms"] - base["step_kernel_ms"]) / max(launches, 1)
+    algo_bytes = 8 * (6 * 2 + 1) * N
+    achieved = algo_bytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    roofline = {"bound": "hbm", "kernel": "k_step<2> (fused forces+ADAM)",
+                "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
+                "avg_launch_ms": round(kms, 5), "algorithmic_bytes_per_launch": algo_bytes,
+                "list_build_ms_per_step": round(
+                    (info["build_ms"] - base["build_ms"]) / max(done, 1), 5)}
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline (reference engine on host cores)...")
+        try:
+            cpu = cpu_baseline(inp, cfg, N, args.cpu_budget)
+        except Exception as e:  # the GPU figure stands without the baseline
+            log(f"[bench] cpu baseline failed: {e}")
+    line = {
+        "metric": METRIC, "value": value, "unit": "particle-updates/s", "n_gpus": 1,
+        "steps": done, "warmup": W, "ms_per_step": 1e3 * dt / max(done, 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "plain method single large packing: 2-D, N=1,000,000 disks, "
+                               "periodic square box, log-normal diameters (mean 1.0, std 0.2), "
+                               "initial phi 0.10, seed 123, steps 1..K",
+                   "N": N, "ndim": 2, "parallelism": "single packing, one GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -63,6 +63,7 @@ typedef struct rcp_run_config {
     size_t reorder_interval;      /* reference default 2000; 0 disables */
     int has_initial_phi_target; double initial_phi_target;
     int exact_reductions;         /* 1 = reference one-thread summation order */
+    int time_kernels;             /* 1 = HIP-event timing of every fused-step launch */
 } rcp_run_config;
 
 /* Fills the reference defaults (reorder_interval 2000, everything else 0). */
@@ -133,6 +134,12 @@ int rcp_engine_trace(const rcp_engine* e, size_t k, size_t* step, double* phi,
  * (diagnostics: rebuild rate, mean row length). */
 int rcp_engine_stats(const rcp_engine* e, uint64_t* list_builds,
                      uint64_t* list_entries, uint64_t* reorders);
+
+/* Accumulated HIP-event durations (time_kernels = 1): fused-step kernel
+ * launches and neighbour-list builds, in milliseconds.  No reference
+ * counterpart (measurement hook for benchmarks). */
+int rcp_engine_timing(const rcp_engine* e, double* step_ms, uint64_t* step_launches,
+                      double* build_ms, uint64_t* builds);
 
 void rcp_engine_destroy(rcp_engine* e);
 

@@ -27,7 +27,8 @@ RCP_OK, RCP_E_ARG, RCP_E_RUNTIME, RCP_E_HIP, RCP_E_CALLBACK, RCP_E_NOMEM = range
 SYMBOLS = (
     "rcp_last_error", "rcp_device_info", "rcp_run_config_init", "rcp_engine_create",
     "rcp_engine_run", "rcp_engine_finish", "rcp_engine_sync", "rcp_engine_info",
-    "rcp_engine_result", "rcp_engine_trace", "rcp_engine_stats", "rcp_engine_destroy",
+    "rcp_engine_result", "rcp_engine_trace", "rcp_engine_stats", "rcp_engine_timing",
+    "rcp_engine_destroy",
     "rcp_run_packing_observed", "rcp_forces", "rcp_pairs", "rcp_mean_force",
     "rcp_adam_positions", "rcp_reorder_perm", "rcp_scale_diameters", "rcp_init_config_init",
     "rcp_initialize_particles",
@@ -48,6 +49,7 @@ class RunConfig(C.Structure):
         ("reorder_interval", C.c_size_t),
         ("has_initial_phi_target", C.c_int), ("initial_phi_target", C.c_double),
         ("exact_reductions", C.c_int),
+        ("time_kernels", C.c_int),
     ]
 
 
@@ -86,6 +88,9 @@ def lib():
         L.rcp_engine_result.argtypes = [C.c_void_p] + [C.c_void_p] * 12
         L.rcp_engine_stats.argtypes = [C.c_void_p, C.POINTER(C.c_uint64),
                                        C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        L.rcp_engine_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double),
+                                        C.POINTER(C.c_uint64), C.POINTER(C.c_double),
+                                        C.POINTER(C.c_uint64)]
         L.rcp_engine_destroy.argtypes = [C.c_void_p]
         L.rcp_engine_destroy.restype = None
         L.rcp_forces.argtypes = [C.c_size_t, C.c_size_t, _dp, _dp, _dp, _i8p, C.c_double,
@@ -179,7 +184,8 @@ class Engine:
 
     def __init__(self, positions, diameters, box=None, walls=None, fixed=None, fix_height=False,
                  max_steps=0, mu=None, fix_diameter=False, target_phi=None,
-                 reorder_interval=2000, initial_phi_target=None, exact=False, neighbor_max=0):
+                 reorder_interval=2000, initial_phi_target=None, exact=False, neighbor_max=0,
+                 time_kernels=False):
         self._x = _f64(positions)
         self.N, self.ndim = self._x.shape
         self._D = _f64(diameters)
@@ -205,6 +211,7 @@ class Engine:
         c.has_initial_phi_target = int(initial_phi_target is not None)
         c.initial_phi_target = float(initial_phi_target or 0.55)
         c.exact_reductions = int(exact)
+        c.time_kernels = int(time_kernels)
         self._cfg = c
         h = C.c_void_p()
         _check(lib().rcp_engine_create(C.byref(c), C.byref(h)))
@@ -234,6 +241,14 @@ class Engine:
         b, e, r = C.c_uint64(), C.c_uint64(), C.c_uint64()
         _check(lib().rcp_engine_stats(self._h, C.byref(b), C.byref(e), C.byref(r)))
         return {"list_builds": b.value, "list_entries": e.value, "reorders": r.value}
+
+    def timing(self):
+        sm, bm = C.c_double(), C.c_double()
+        sn, bn = C.c_uint64(), C.c_uint64()
+        _check(lib().rcp_engine_timing(self._h, C.byref(sm), C.byref(sn), C.byref(bm),
+                                       C.byref(bn)))
+        return {"step_ms": sm.value, "step_launches": sn.value, "build_ms": bm.value,
+                "builds": bn.value}
 
     def result(self) -> dict:
         i = self.info()

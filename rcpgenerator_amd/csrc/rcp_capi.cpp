@@ -312,6 +312,22 @@ int rcp_engine_stats(const rcp_engine* e, uint64_t* list_builds, uint64_t* list_
     });
 }
 
+int rcp_engine_timing(const rcp_engine* e, double* step_ms, uint64_t* step_launches,
+                      double* build_ms, uint64_t* builds)
+{
+    return guarded([&]() {
+        if (!e) throw rcp::ArgError("rcp: null engine");
+        double sm = 0.0, bm = 0.0;
+        uint64_t sn = 0, bn = 0;
+        e->e->timing(sm, sn, bm, bn);
+        if (step_ms) *step_ms = sm;
+        if (step_launches) *step_launches = sn;
+        if (build_ms) *build_ms = bm;
+        if (builds) *builds = bn;
+        return RCP_OK;
+    });
+}
+
 void rcp_engine_destroy(rcp_engine* e)
 {
     try {

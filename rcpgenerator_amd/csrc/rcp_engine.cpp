@@ -200,6 +200,8 @@ Engine::Engine(const rcp_run_config& c)
     }
     geo_.curved_T = curved_T_;
     hip_check(hipStreamCreate(&stream_), "hipStreamCreate");
+    if (opt_.time_kernels)
+        for (auto& ev : ev_) hip_check(hipEventCreate(&ev), "hipEventCreate");
     alloc_device();
     hip_check(hipMemcpyAsync(x_[cur_], x.data(), N_ * nd_ * sizeof(double),
                              hipMemcpyHostToDevice, stream_), "upload x");
@@ -225,6 +227,8 @@ Engine::~Engine()
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (void* p : allocs_) dev_free(p);
     if (h_result_) (void)hipHostFree(h_result_);
+    for (auto& ev : ev_)
+        if (ev) (void)hipEventDestroy(ev);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -382,7 +386,16 @@ void Engine::build_lists()
     la.t_shell = t_list_;
     la.half = 0;
     configure_grid(Dmax + t_list_, la.grid);
+    if (opt_.time_kernels) hip_check(hipEventRecord(ev_[2], stream_), "event");
     build_rows(la);
+    if (opt_.time_kernels) {
+        hip_check(hipEventRecord(ev_[3], stream_), "event");
+        hip_check(hipEventSynchronize(ev_[3]), "event sync");
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, ev_[2], ev_[3]), "event time");
+        t_build_ms_ += ms;
+        ++t_build_n_;
+    }
     hip_check(hipMemcpyAsync(x_anchor_, x_[cur_], N_ * nd_ * sizeof(double),
                              hipMemcpyDeviceToDevice, stream_), "anchor copy");
     kappa_b_ = kD;
@@ -487,12 +500,20 @@ Engine::Scalars Engine::step_device(bool do_adam, double alpha, double inv_b1, d
         a.mag_out = mag_;
         a.contrib = contrib_;
     }
+    if (opt_.time_kernels) hip_check(hipEventRecord(ev_[0], stream_), "event");
     if (launch_step(a, stream_)) throw HipFailure("step launch");
+    if (opt_.time_kernels) hip_check(hipEventRecord(ev_[1], stream_), "event");
     if (opt_.exact_reductions && launch_exact_reduce(a, stream_))
         throw HipFailure("exact reduce launch");
     hip_check(hipMemcpyAsync(h_result_, result_, S_NUM * sizeof(double), hipMemcpyDeviceToHost,
                              stream_), "scalars download");
     hip_check(hipStreamSynchronize(stream_), "step sync");
+    if (opt_.time_kernels) {
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "event time");
+        t_step_ms_ += ms;
+        ++t_step_n_;
+    }
     Scalars s;
     for (int k = 0; k < S_NUM; ++k) s.v[k] = h_result_[k];
     return s;

@@ -76,6 +76,13 @@ public:
     uint64_t list_builds() const { return list_builds_; }
     uint64_t list_entries() const { return list_entries_; }
     uint64_t reorders() const { return reorders_; }
+    void timing(double& step_ms, uint64_t& steps, double& build_ms, uint64_t& builds) const
+    {
+        step_ms = t_step_ms_;
+        steps = t_step_n_;
+        build_ms = t_build_ms_;
+        builds = t_build_n_;
+    }
 
 private:
     struct Scalars { double v[S_NUM]; };
@@ -144,6 +151,11 @@ private:
     bool need_build_ = true;
     uint64_t list_builds_ = 0, list_entries_ = 0, reorders_ = 0;
     size_t nnz_ = 0;
+
+    // kernel timing (HIP events on stream_)
+    hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
+    double t_step_ms_ = 0.0, t_build_ms_ = 0.0;
+    uint64_t t_step_n_ = 0, t_build_n_ = 0;
 
     // device
     Geometry geo_;

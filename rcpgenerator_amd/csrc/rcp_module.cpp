@@ -145,9 +145,11 @@ void parse_run(const py::dict& input, const py::dict& config, const py::dict& op
         c.has_initial_phi_target = 1;
         c.initial_phi_target = options["initial_phi_target"].cast<double>();
     }
-    bool exact = false;
+    bool exact = false, time_kernels = false;
     take(options, "exact_reductions", exact);
+    take(options, "time_kernels", time_kernels);
     c.exact_reductions = exact;
+    c.time_kernels = time_kernels;
 }
 
 py::dict result_dict(rcp_engine* e)
@@ -344,6 +346,13 @@ public:
         d["list_builds"] = builds;
         d["list_entries"] = entries;
         d["reorders"] = reorders;
+        double sm = 0, bm = 0;
+        uint64_t sn = 0, bn = 0;
+        rcp_engine_timing(h_.e, &sm, &sn, &bm, &bn);
+        d["step_kernel_ms"] = sm;
+        d["step_kernel_launches"] = sn;
+        d["build_ms"] = bm;
+        d["builds_timed"] = bn;
         return d;
     }
 
