@@ -64,6 +64,10 @@ typedef struct rcp_run_config {
     int has_initial_phi_target; double initial_phi_target;
     int exact_reductions;         /* 1 = reference one-thread summation order */
     int time_kernels;             /* 1 = HIP-event timing of every fused-step launch */
+    int neighbor_mode;            /* 0 = automatic (lists when they live long enough,
+                                     per-step cell scan otherwise), 1 = lists only,
+                                     2 = per-step cell scan only (not with
+                                     exact_reductions) */
 } rcp_run_config;
 
 /* Fills the reference defaults (reorder_interval 2000, everything else 0). */
@@ -130,10 +134,13 @@ int rcp_engine_trace(const rcp_engine* e, size_t k, size_t* step, double* phi,
                      double* force, double* energy, double* max_min_dist,
                      double* positions, double* diameters, int* has_positions);
 
-/* Number of neighbour-list builds and total list entries so far
- * (diagnostics: rebuild rate, mean row length). */
+/* Number of neighbour-list builds and total list entries so far, reference
+ * reorders, steps taken without a list (per-step cell scan) and steps in
+ * which some particle had more contacts than the scan's buffer (those take
+ * the slower selection path).  Diagnostics only. */
 int rcp_engine_stats(const rcp_engine* e, uint64_t* list_builds,
-                     uint64_t* list_entries, uint64_t* reorders);
+                     uint64_t* list_entries, uint64_t* reorders,
+                     uint64_t* direct_steps, uint64_t* overflow_steps);
 
 /* Accumulated HIP-event durations (time_kernels = 1): fused-step kernel
  * launches and neighbour-list builds, in milliseconds.  No reference

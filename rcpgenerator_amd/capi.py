@@ -50,6 +50,7 @@ class RunConfig(C.Structure):
         ("has_initial_phi_target", C.c_int), ("initial_phi_target", C.c_double),
         ("exact_reductions", C.c_int),
         ("time_kernels", C.c_int),
+        ("neighbor_mode", C.c_int),
     ]
 
 
@@ -86,8 +87,7 @@ def lib():
         L.rcp_engine_sync.argtypes = [C.c_void_p]
         L.rcp_engine_info.argtypes = [C.c_void_p, C.POINTER(ResultInfo)]
         L.rcp_engine_result.argtypes = [C.c_void_p] + [C.c_void_p] * 12
-        L.rcp_engine_stats.argtypes = [C.c_void_p, C.POINTER(C.c_uint64),
-                                       C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        L.rcp_engine_stats.argtypes = [C.c_void_p] + [C.POINTER(C.c_uint64)] * 5
         L.rcp_engine_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double),
                                         C.POINTER(C.c_uint64), C.POINTER(C.c_double),
                                         C.POINTER(C.c_uint64)]
@@ -185,7 +185,7 @@ class Engine:
     def __init__(self, positions, diameters, box=None, walls=None, fixed=None, fix_height=False,
                  max_steps=0, mu=None, fix_diameter=False, target_phi=None,
                  reorder_interval=2000, initial_phi_target=None, exact=False, neighbor_max=0,
-                 time_kernels=False):
+                 time_kernels=False, neighbor_mode=0):
         self._x = _f64(positions)
         self.N, self.ndim = self._x.shape
         self._D = _f64(diameters)
@@ -212,6 +212,7 @@ class Engine:
         c.initial_phi_target = float(initial_phi_target or 0.55)
         c.exact_reductions = int(exact)
         c.time_kernels = int(time_kernels)
+        c.neighbor_mode = int(neighbor_mode)
         self._cfg = c
         h = C.c_void_p()
         _check(lib().rcp_engine_create(C.byref(c), C.byref(h)))
@@ -238,9 +239,11 @@ class Engine:
         return i
 
     def stats(self):
-        b, e, r = C.c_uint64(), C.c_uint64(), C.c_uint64()
-        _check(lib().rcp_engine_stats(self._h, C.byref(b), C.byref(e), C.byref(r)))
-        return {"list_builds": b.value, "list_entries": e.value, "reorders": r.value}
+        b, e, r, d, o = (C.c_uint64() for _ in range(5))
+        _check(lib().rcp_engine_stats(self._h, C.byref(b), C.byref(e), C.byref(r), C.byref(d),
+                                      C.byref(o)))
+        return {"list_builds": b.value, "list_entries": e.value, "reorders": r.value,
+                "direct_steps": d.value, "overflow_steps": o.value}
 
     def timing(self):
         sm, bm = C.c_double(), C.c_double()

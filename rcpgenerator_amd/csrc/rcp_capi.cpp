@@ -92,75 +92,13 @@ rcp::Geometry make_geometry(size_t N, size_t ndim, const double* box, const int8
     return g;
 }
 
-// Cell-list rows on the device for positions x (device) and diameters D0
-// (device, kappa = 1): full rows (half = 0) or j > i rows (half = 1).
-struct DeviceRows {
-    std::unique_ptr<DBuf<uint32_t>> cell_of, slot, items, row_count, row_start, cell_count,
-        cell_start, nbr;
-    std::unique_ptr<DBuf<char>> scan_tmp;
-    uint32_t nnz = 0;
+// One-shot device state for the kernel-level calls: inputs uploaded in
+// reference order (tags = identity), binned for cells at least `reach` wide.
+struct OneShot {
+    Stream st;
+    rcp::DeviceState S;
+    OneShot(size_t N, size_t ndim, bool exact) : S((int)N, (int)ndim, false, exact, st.s) {}
 };
-
-void device_rows(const rcp::Geometry& g, const double* x, const double* D0, double Dmax,
-                 double t_shell, int half, DeviceRows& r, hipStream_t s)
-{
-    const size_t N = (size_t)g.N;
-    rcp::ListArgs la;
-    la.g = g;
-    la.x = x;
-    la.D0 = D0;
-    la.kappa = 1.0;
-    la.t_shell = t_shell;
-    la.half = half;
-    const double edge = (Dmax + t_shell) * (1.0 + 1e-6);
-    long long total = 1;
-    for (int d = 0; d < g.ndim; ++d) {
-        long long n = (edge > 0.0) ? (long long)std::floor(g.box[d] / edge) : 1;
-        if (n < 3) n = 1;
-        if (n > (1 << 20)) n = 1 << 20;
-        la.grid.nc[d] = (int)n;
-        total *= n;
-    }
-    const long long cap = std::max<long long>(64, 2 * (long long)N);
-    while (total > cap) {
-        int dm = 0;
-        for (int d = 1; d < g.ndim; ++d)
-            if (la.grid.nc[d] > la.grid.nc[dm]) dm = d;
-        int n = la.grid.nc[dm] / 2;
-        if (n < 3) n = 1;
-        total = total / la.grid.nc[dm] * n;
-        la.grid.nc[dm] = n;
-    }
-    for (int d = 0; d < g.ndim; ++d) la.grid.cell_scale[d] = la.grid.nc[d] / g.box[d];
-    la.grid.total = total;
-    r.cell_of.reset(new DBuf<uint32_t>(N));
-    r.slot.reset(new DBuf<uint32_t>(N));
-    r.items.reset(new DBuf<uint32_t>(N));
-    r.row_count.reset(new DBuf<uint32_t>(N + 1));
-    r.row_start.reset(new DBuf<uint32_t>(N + 1));
-    r.cell_count.reset(new DBuf<uint32_t>((size_t)total + 1));
-    r.cell_start.reset(new DBuf<uint32_t>((size_t)total + 1));
-    const size_t tb = rcp::scan_tmp_bytes(std::max<size_t>((size_t)total, N) + 8);
-    r.scan_tmp.reset(new DBuf<char>(tb));
-    la.cell_of = r.cell_of->p;
-    la.slot = r.slot->p;
-    la.items = r.items->p;
-    la.row_count = r.row_count->p;
-    la.row_start = r.row_start->p;
-    la.cell_count = r.cell_count->p;
-    la.cell_start = r.cell_start->p;
-    if (rcp::launch_cells(la, r.scan_tmp->p, tb, s)) throw rcp::HipFailure("cells launch");
-    if (rcp::launch_rows_count(la, s)) throw rcp::HipFailure("rows count launch");
-    if (rcp::launch_scan_u32(r.row_count->p, r.row_start->p, N, r.scan_tmp->p, s))
-        throw rcp::HipFailure("scan launch");
-    rcp::hip_check(hipMemcpyAsync(&r.nnz, r.row_start->p + N, sizeof(uint32_t),
-                                  hipMemcpyDeviceToHost, s), "nnz");
-    rcp::hip_check(hipStreamSynchronize(s), "sync");
-    r.nbr.reset(new DBuf<uint32_t>((size_t)r.nnz + 1));
-    la.nbr = r.nbr->p;
-    la.nbr_cap = (size_t)r.nnz + 1;
-    if (rcp::launch_rows_fill(la, s)) throw rcp::HipFailure("rows fill launch");
-}
 
 }  // namespace
 
@@ -301,13 +239,15 @@ int rcp_engine_trace(const rcp_engine* e, size_t k, size_t* step, double* phi, d
 }
 
 int rcp_engine_stats(const rcp_engine* e, uint64_t* list_builds, uint64_t* list_entries,
-                     uint64_t* reorders)
+                     uint64_t* reorders, uint64_t* direct_steps, uint64_t* overflow_steps)
 {
     return guarded([&]() {
         if (!e) throw rcp::ArgError("rcp: null engine");
         if (list_builds) *list_builds = e->e->list_builds();
         if (list_entries) *list_entries = e->e->list_entries();
         if (reorders) *reorders = e->e->reorders();
+        if (direct_steps) *direct_steps = e->e->direct_steps();
+        if (overflow_steps) *overflow_steps = e->e->overflow_steps();
         return RCP_OK;
     });
 }
@@ -367,48 +307,55 @@ int rcp_forces(size_t N, size_t ndim, const double* x, const double* D, const do
     return guarded([&]() {
         check_dims(N, ndim);
         rcp::Geometry g = make_geometry(N, ndim, box, walls);
-        Stream st;
-        DBuf<double> dx(N * ndim), dD(N), dF(N * ndim), dmag(N), dpart(((N + 255) / 256 + 1) * rcp::S_NUM),
-            dres(rcp::S_NUM);
-        DBuf<unsigned int> dcnt(1);
-        dx.up(x, N * ndim, st.s);
-        dD.up(D, N, st.s);
-        rcp::hip_check(hipMemsetAsync(dcnt.p, 0, sizeof(unsigned int), st.s), "memset");
-        std::unique_ptr<DBuf<double>> drk;
-        if (rk) {
-            drk.reset(new DBuf<double>(N));
-            drk->up(rk, N, st.s);
-        }
+        OneShot o(N, ndim, exact != 0);
+        rcp::DeviceState& S = o.S;
+        S.upload(x, D, nullptr);
         const double Dmax = *std::max_element(D, D + N);
         const double Dmin = *std::min_element(D, D + N);
         // contacts need d < (Di+Dj)/2; a small positive shell keeps every
-        // contact in the list (the kernel re-tests d2 < r^2 exactly)
-        DeviceRows rows;
-        device_rows(g, dx.p, dD.p, Dmax, std::max(0.05 * Dmin, 1e-12), 0, rows, st.s);
-        std::unique_ptr<DBuf<double>> dcontrib;
-        if (exact) dcontrib.reset(new DBuf<double>(((size_t)rows.nnz + 1) * 4));
+        // contact in the rows (the kernel re-tests d2 < r^2 exactly)
+        const double shell = std::max(0.05 * Dmin, 1e-12);
+        S.rebin(g, Dmax + shell);
+        S.build_rows(g, 1.0, shell, 0, 0);
+        if (rk) {
+            S.rk = S.alloc<double>(N);
+            rcp::hip_check(hipMemcpyAsync(S.tmp, rk, N * sizeof(double), hipMemcpyHostToDevice,
+                                          S.s), "upload rk");
+            if (rcp::launch_from_ref_f64(S.tmp, S.rk, S.tag, (int)N, 1, S.s))
+                throw rcp::HipFailure("gather launch");
+        }
         rcp::StepArgs a;
         a.g = g;
-        a.x_in = dx.p;
-        a.D0 = dD.p;
-        a.row_start = rows.row_start->p;
-        a.nbr = rows.nbr->p;
+        a.x_in = S.x;
+        a.D0 = S.D0;
+        a.tag = S.tag;
+        a.row_start = S.row_start;
+        a.nbr = S.nbr;
+        a.grid = S.grid;
+        a.cell_start = S.cell_start;
         a.kappa = 1.0;
         a.use_rk = rk ? 1 : 0;
-        a.rk = rk ? drk->p : nullptr;
+        a.rk = S.rk;
         a.do_adam = 0;
-        a.F_out = dF.p;
-        a.mag_out = dmag.p;
-        a.contrib = exact ? dcontrib->p : nullptr;
-        a.partials = dpart.p;
-        a.counter = dcnt.p;
-        a.result = dres.p;
-        if (rcp::launch_step(a, st.s)) throw rcp::HipFailure("step launch");
-        if (exact && rcp::launch_exact_reduce(a, st.s)) throw rcp::HipFailure("exact launch");
+        a.F_out = S.anchor;
+        a.inv_tag = S.inv_tag;
+        a.partials = S.partials;
+        a.counter = S.counter;
+        a.result = S.result;
+        if (exact) {
+            a.mag_out = S.mag;
+            a.contrib = S.contrib;
+            if (rcp::launch_exact_step(a, S.ref_count, S.ref_start, S.ref_terms, S.scan_tmp, S.s))
+                throw rcp::HipFailure("exact step launch");
+        } else if (rcp::launch_step_list(a, S.s)) {
+            throw rcp::HipFailure("step launch");
+        }
         double r[rcp::S_NUM];
-        dres.down(r, rcp::S_NUM, st.s);
-        dF.down(F, N * ndim, st.s);
-        st.sync();
+        rcp::hip_check(hipMemcpyAsync(r, S.result, sizeof(r), hipMemcpyDeviceToHost, S.s),
+                       "download");
+        std::vector<double> Fh;
+        S.to_ref(S.anchor, (int)ndim, Fh);
+        std::memcpy(F, Fh.data(), N * ndim * sizeof(double));
         const double count = 1.0 + r[rcp::S_CNT];
         scalars[0] = std::pow(r[rcp::S_U] / count, 2);
         scalars[1] = r[rcp::S_MAXOV];
@@ -426,22 +373,33 @@ int rcp_pairs(size_t N, size_t ndim, const double* x, const double* D, const dou
     return guarded([&]() {
         check_dims(N, ndim);
         rcp::Geometry g = make_geometry(N, ndim, box, nullptr);
-        Stream st;
-        DBuf<double> dx(N * ndim), dD(N);
-        dx.up(x, N * ndim, st.s);
-        dD.up(D, N, st.s);
+        OneShot o(N, ndim, false);
+        rcp::DeviceState& S = o.S;
+        S.upload(x, D, nullptr);
         std::vector<double> Dv(D, D + N);
         const double t_shell = rcp::reference_shell(Dv);
         const double Dmax = *std::max_element(D, D + N);
-        DeviceRows rows;
-        device_rows(g, dx.p, dD.p, Dmax, t_shell, 1, rows, st.s);
-        *npairs = rows.nnz;
+        S.rebin(g, Dmax + t_shell);
+        S.build_rows(g, 1.0, t_shell, 1, 0);
+        *npairs = S.nnz;
+        // rows re-indexed to the input order: owner by tag, partners as tags
+        uint32_t* ref_count = S.alloc<uint32_t>(N + 1);
+        uint32_t* ref_start = S.alloc<uint32_t>(N + 1);
+        uint32_t* ref_nbr = S.alloc<uint32_t>((size_t)S.nnz + 1);
+        if (rcp::launch_rows_to_ref_count(S.row_start, S.tag, ref_count, (int)N, S.s) ||
+            rcp::launch_scan_u32(ref_count, ref_start, N, S.scan_tmp, S.s) ||
+            rcp::launch_rows_to_ref_fill(S.row_start, S.nbr, S.tag, ref_start, ref_nbr, (int)N,
+                                         S.s))
+            throw rcp::HipFailure("rows launch");
         std::vector<uint32_t> rs(N + 1);
-        rows.row_start->down(rs.data(), N + 1, st.s);
-        if (nbr && nbr_capacity >= rows.nnz) rows.nbr->down(nbr, rows.nnz, st.s);
-        st.sync();
+        rcp::hip_check(hipMemcpyAsync(rs.data(), ref_start, (N + 1) * sizeof(uint32_t),
+                                      hipMemcpyDeviceToHost, S.s), "download");
+        if (nbr && nbr_capacity >= S.nnz && S.nnz)
+            rcp::hip_check(hipMemcpyAsync(nbr, ref_nbr, (size_t)S.nnz * sizeof(uint32_t),
+                                          hipMemcpyDeviceToHost, S.s), "download");
+        S.sync();
         for (size_t i = 0; i <= N; ++i) row_off[i] = rs[i];
-        if (nbr && nbr_capacity < rows.nnz) throw rcp::ArgError("rcp_pairs: nbr capacity too small");
+        if (nbr && nbr_capacity < S.nnz) throw rcp::ArgError("rcp_pairs: nbr capacity too small");
         return RCP_OK;
     });
 }
@@ -454,7 +412,7 @@ int rcp_mean_force(size_t N, size_t ndim, const double* F, double* out)
         rcp::Geometry g;
         g.N = (int)N;
         g.ndim = (int)ndim;
-        DBuf<double> dF(N * ndim), dmag(N), dpart(((N + 255) / 256 + 1) * rcp::S_NUM),
+        DBuf<double> dF(N * ndim), dmag(N), dpart((rcp::step_partial_blocks(N) + 1) * rcp::S_NUM),
             dres(rcp::S_NUM);
         DBuf<unsigned int> dcnt(1);
         dF.up(F, N * ndim, st.s);
@@ -501,20 +459,21 @@ int rcp_reorder_perm(size_t N, size_t ndim, const double* x, const double* box, 
 {
     return guarded([&]() {
         check_dims(N, ndim);
+        rcp::Geometry g = make_geometry(N, ndim, box, nullptr);
         Stream st;
-        DBuf<double> dx(N * ndim), dbox(ndim);
+        DBuf<double> dx(N * ndim);
         dx.up(x, N * ndim, st.s);
         DBuf<unsigned long long> k(N), k2(N);
         DBuf<uint32_t> v(N), v2(N), hist(rcp::sort_hist_entries(N));
         DBuf<char> tmp(rcp::scan_tmp_bytes(rcp::sort_hist_entries(N) + 8));
-        if (rcp::launch_reorder_keys(dx.p, (int)N, (int)ndim, box, k.p, v.p, st.s))
-            throw rcp::HipFailure("keys launch");
         rcp::SortTmp t;
+        t.keys = k.p;
         t.keys_alt = k2.p;
+        t.vals = v.p;
         t.vals_alt = v2.p;
         t.hist = hist.p;
         t.scan_tmp = tmp.p;
-        if (rcp::launch_radix_sort(k.p, v.p, N, 64, t, st.s)) throw rcp::HipFailure("sort launch");
+        if (rcp::launch_reorder_perm(g, dx.p, v.p, t, st.s)) throw rcp::HipFailure("sort launch");
         v.down(perm, N, st.s);
         st.sync();
         return RCP_OK;

@@ -1,6 +1,6 @@
 // Host side of the MI355X packing engine: the run driver and schedule
 // controller of rcpgenerator::run_packing_observed, restated for a device-
-// resident state, plus the C ABI declared in include/rcp_amd.h.
+// resident state kept in working (cell) order.
 //
 // Compiled with g++ (the compiler the reference is built with) so the
 // scalar controller arithmetic and libm calls round exactly as the
@@ -28,11 +28,8 @@ thread_local std::string g_last_error;
 
 void hip_check(hipError_t e, const char* what)
 {
-    if (e != hipSuccess)
-        throw HipFailure(std::string(what) + ": " + hipGetErrorString(e));
+    if (e != hipSuccess) throw HipFailure(std::string(what) + ": " + hipGetErrorString(e));
 }
-
-// ------------------------------------------------------------ DevArena
 
 void* dev_alloc(size_t bytes)
 {
@@ -82,15 +79,295 @@ double reference_shell(std::vector<double> D)
     return std::min(std::max(median * 1.02, Dmin * 2.25), Dmax) * 0.5;
 }
 
+void make_grid(const double* box, int ndim, double reach, long long cap, CellGrid& grid)
+{
+    const double edge = reach * (1.0 + 1e-6);
+    long long total = 1;
+    for (int d = 0; d < ndim; ++d) {
+        long long n = (edge > 0.0) ? (long long)std::floor(box[d] / edge) : 1;
+        if (n < 3) n = 1;
+        if (n > (1 << 20)) n = 1 << 20;
+        grid.nc[d] = (int)n;
+        total *= n;
+    }
+    // bigger cells stay correct; keep the table near the particle count
+    while (total > cap) {
+        int dm = 0;
+        for (int d = 1; d < ndim; ++d)
+            if (grid.nc[d] > grid.nc[dm]) dm = d;
+        int n = grid.nc[dm] / 2;
+        if (n < 3) n = 1;
+        total = total / grid.nc[dm] * n;
+        grid.nc[dm] = n;
+    }
+    for (int d = 0; d < ndim; ++d) grid.cell_scale[d] = grid.nc[d] / box[d];
+    for (int d = ndim; d < kMaxDim; ++d) {
+        grid.nc[d] = 1;
+        grid.cell_scale[d] = 0.0;
+    }
+    grid.total = total;
+}
+
+// ------------------------------------------------------------ DeviceState
+
+void DeviceState::release(void* p)
+{
+    auto it = std::find(allocs_.begin(), allocs_.end(), p);
+    if (it != allocs_.end()) {
+        dev_free(p);
+        allocs_.erase(it);
+    }
+}
+
+void DeviceState::ensure_scan(size_t n)
+{
+    const size_t bytes = scan_tmp_bytes(n);
+    if (bytes <= scan_bytes) return;
+    if (scan_tmp) release(scan_tmp);
+    scan_tmp = alloc<char>(bytes);
+    scan_bytes = bytes;
+    sort.scan_tmp = scan_tmp;
+}
+
+DeviceState::DeviceState(int N_, int nd_, bool with_fixed, bool exact_, hipStream_t s_)
+    : N(N_), nd(nd_), has_fixed_array(with_fixed), exact(exact_), s(s_)
+{
+    try {
+        const size_t Nd = (size_t)N * nd;
+        x = alloc<double>(Nd);
+        x_alt = alloc<double>(Nd);
+        m = alloc<double>(Nd);
+        m_alt = alloc<double>(Nd);
+        v = alloc<double>(Nd);
+        v_alt = alloc<double>(Nd);
+        anchor = alloc<double>(Nd);
+        tmp = alloc<double>(Nd);
+        tmp2 = alloc<double>(N);
+        D0 = alloc<double>(N);
+        D0_alt = alloc<double>(N);
+        tag = alloc<uint32_t>(N);
+        tag_alt = alloc<uint32_t>(N);
+        inv_tag = alloc<uint32_t>(N);
+        tag_tmp = alloc<uint32_t>(N);
+        origin = alloc<unsigned long long>(N);
+        origin_alt = alloc<unsigned long long>(N);
+        u64tmp = alloc<unsigned long long>(N);
+        i8tmp = alloc<int8_t>(N);
+        if (has_fixed_array) {
+            fixed = alloc<int8_t>(N);
+            fixed_alt = alloc<int8_t>(N);
+        }
+        cell_of = alloc<uint32_t>(N);
+        slot = alloc<uint32_t>(N);
+        perm = alloc<uint32_t>(N);
+        row_count = alloc<uint32_t>((size_t)N + 1);
+        row_start = alloc<uint32_t>((size_t)N + 1);
+        sort.keys = alloc<unsigned long long>(N);
+        sort.keys_alt = alloc<unsigned long long>(N);
+        sort.vals = alloc<uint32_t>(N);
+        sort.vals_alt = alloc<uint32_t>(N);
+        sort.hist = alloc<uint32_t>(sort_hist_entries(N));
+        partials = alloc<double>((step_partial_blocks(N) + 1) * S_NUM);
+        result = alloc<double>(S_NUM);
+        counter = alloc<unsigned int>(1);
+        if (exact) {
+            mag = alloc<double>(N);
+            ref_count = alloc<uint32_t>((size_t)N + 1);
+            ref_start = alloc<uint32_t>((size_t)N + 1);
+        }
+        hip_check(hipMemsetAsync(counter, 0, sizeof(unsigned int), s), "zero counter");
+        hip_check(hipHostMalloc((void**)&h_result, S_NUM * sizeof(double),
+                                hipHostMallocDefault), "hipHostMalloc");
+        ensure_scan(std::max(sort_hist_entries(N), (size_t)N) + 8);
+    } catch (...) {
+        for (void* p : allocs_) dev_free(p);
+        allocs_.clear();
+        throw;
+    }
+}
+
+DeviceState::~DeviceState()
+{
+    if (s) (void)hipStreamSynchronize(s);
+    for (void* p : allocs_) dev_free(p);
+    if (h_result) (void)hipHostFree(h_result);
+}
+
+void DeviceState::sync() { hip_check(hipStreamSynchronize(s), "sync"); }
+
+void DeviceState::upload(const double* xh, const double* D0h, const int8_t* fixedh)
+{
+    hip_check(hipMemcpyAsync(x, xh, (size_t)N * nd * sizeof(double), hipMemcpyHostToDevice, s),
+              "upload x");
+    hip_check(hipMemcpyAsync(D0, D0h, (size_t)N * sizeof(double), hipMemcpyHostToDevice, s),
+              "upload D0");
+    if (has_fixed_array)
+        hip_check(hipMemcpyAsync(fixed, fixedh, (size_t)N, hipMemcpyHostToDevice, s),
+                  "upload fixed");
+    hip_check(hipMemsetAsync(m, 0, (size_t)N * nd * sizeof(double), s), "zero m");
+    hip_check(hipMemsetAsync(v, 0, (size_t)N * nd * sizeof(double), s), "zero v");
+    if (launch_iota_u32(tag, N, s) || launch_iota_u32(inv_tag, N, s) ||
+        launch_iota_u64(origin, N, s))
+        throw HipFailure("iota launch");
+    sync();
+}
+
+void DeviceState::rebin(const Geometry& g, double reach)
+{
+    make_grid(g.box, nd, reach, std::max<long long>(64, 2 * (long long)N), grid);
+    const size_t total = (size_t)grid.total;
+    if (total + 1 > cell_cap) {
+        if (cell_count) {
+            release(cell_count);
+            release(cell_start);
+        }
+        cell_cap = total + 1;
+        cell_count = alloc<uint32_t>(cell_cap);
+        cell_start = alloc<uint32_t>(cell_cap);
+    }
+    ensure_scan(std::max(total, (size_t)N) + 8);
+    BinArgs b;
+    b.g = g;
+    b.grid = grid;
+    b.x = x;
+    b.m = m;
+    b.v = v;
+    b.D0 = D0;
+    b.tag = tag;
+    b.origin = origin;
+    b.fixed = fixed;
+    b.x_o = x_alt;
+    b.m_o = m_alt;
+    b.v_o = v_alt;
+    b.D0_o = D0_alt;
+    b.tag_o = tag_alt;
+    b.origin_o = origin_alt;
+    b.fixed_o = fixed_alt;
+    b.inv_tag = inv_tag;
+    b.cell_of = cell_of;
+    b.slot = slot;
+    b.cell_count = cell_count;
+    b.cell_start = cell_start;
+    b.perm = perm;
+    if (launch_rebin(b, scan_tmp, s)) throw HipFailure("rebin launch");
+    std::swap(x, x_alt);
+    std::swap(m, m_alt);
+    std::swap(v, v_alt);
+    std::swap(D0, D0_alt);
+    std::swap(tag, tag_alt);
+    std::swap(origin, origin_alt);
+    std::swap(fixed, fixed_alt);
+    binned = true;
+}
+
+void DeviceState::build_rows(const Geometry& g, double kappa, double t_shell, int half,
+                             int sortwalk_rc)
+{
+    ListArgs la;
+    la.g = g;
+    la.grid = grid;
+    la.x = x;
+    la.D0 = D0;
+    la.fixed = fixed;
+    la.tag = tag;
+    la.cell_start = cell_start;
+    la.kappa = kappa;
+    la.t_shell = t_shell;
+    la.half = half;
+    la.sortwalk_rc = sortwalk_rc;
+    la.row_count = row_count;
+    la.row_start = row_start;
+    if (launch_rows_count(la, s)) throw HipFailure("rows count launch");
+    if (launch_scan_u32(row_count, row_start, N, scan_tmp, s)) throw HipFailure("scan launch");
+    uint32_t total = 0;
+    hip_check(hipMemcpyAsync(&total, row_start + N, sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+              "nnz download");
+    sync();
+    if ((size_t)total + 1 > nbr_cap) {
+        if (nbr) release(nbr);
+        if (contrib) {
+            release(contrib);
+            release(ref_terms);
+            contrib = ref_terms = nullptr;
+        }
+        nbr_cap = (size_t)(total * 1.25) + 1024;
+        nbr = alloc<uint32_t>(nbr_cap);
+        if (exact) {
+            contrib = alloc<double>(nbr_cap * 4);
+            ref_terms = alloc<double>(nbr_cap * 4);
+        }
+    }
+    la.nbr = nbr;
+    la.nbr_cap = nbr_cap;
+    if (launch_rows_fill(la, s)) throw HipFailure("rows fill launch");
+    nnz = total;
+}
+
+void DeviceState::reference_reorder(const Geometry& g)
+{
+    if (launch_reference_reorder(g, x, tag, inv_tag, tag_tmp, sort, s))
+        throw HipFailure("reference reorder launch");
+}
+
+void DeviceState::to_ref(const double* src_w, int width, std::vector<double>& out)
+{
+    out.resize((size_t)N * width);
+    if (launch_to_ref_f64(src_w, tmp, tag, N, width, s)) throw HipFailure("scatter launch");
+    hip_check(hipMemcpyAsync(out.data(), tmp, out.size() * sizeof(double), hipMemcpyDeviceToHost,
+                             s), "download");
+    sync();
+}
+
+void DeviceState::d0_ref(std::vector<double>& D0h, std::vector<int8_t>& fixedh)
+{
+    to_ref(D0, 1, D0h);
+    fixedh.clear();
+    if (has_fixed_array) {
+        fixedh.resize(N);
+        if (launch_to_ref_i8(fixed, i8tmp, tag, N, s)) throw HipFailure("scatter launch");
+        hip_check(hipMemcpyAsync(fixedh.data(), i8tmp, N, hipMemcpyDeviceToHost, s), "download");
+        sync();
+    }
+}
+
+void DeviceState::set_d0_ref(const std::vector<double>& D0h, const std::vector<int8_t>& fixedh)
+{
+    hip_check(hipMemcpyAsync(tmp, D0h.data(), (size_t)N * sizeof(double), hipMemcpyHostToDevice,
+                             s), "upload");
+    if (launch_from_ref_f64(tmp, D0, tag, N, 1, s)) throw HipFailure("gather launch");
+    if (has_fixed_array && !fixedh.empty()) {
+        hip_check(hipMemcpyAsync(i8tmp, fixedh.data(), N, hipMemcpyHostToDevice, s), "upload");
+        if (launch_from_ref_i8(i8tmp, fixed, tag, N, s)) throw HipFailure("gather launch");
+    }
+    sync();
+}
+
+void DeviceState::origin_ref(std::vector<uint64_t>& o)
+{
+    o.resize(N);
+    if (launch_to_ref_u64(origin, u64tmp, tag, N, s)) throw HipFailure("scatter launch");
+    hip_check(hipMemcpyAsync(o.data(), u64tmp, (size_t)N * sizeof(uint64_t),
+                             hipMemcpyDeviceToHost, s), "download");
+    sync();
+}
+
+void DeviceState::diameters_ref(double kappa, std::vector<double>& D)
+{
+    if (launch_diameters(D0, fixed, kappa, tmp2, N, s)) throw HipFailure("diameters launch");
+    to_ref(tmp2, 1, D);
+}
+
 // ------------------------------------------------------------- Engine
 
 Engine::Engine(const rcp_run_config& c)
 {
     if (c.N == 0) throw ArgError("rcp: N must be >= 1");
-    if (c.ndim < 1 || c.ndim > (size_t)kMaxDim)
-        throw ArgError("rcp: Ndim must be in 1..8");
+    if (c.ndim < 1 || c.ndim > (size_t)kMaxDim) throw ArgError("rcp: Ndim must be in 1..8");
     if (!c.positions || !c.diameters) throw ArgError("rcp: positions/diameters missing");
     if (c.N > 0x7fffffffull) throw ArgError("rcp: N too large");
+    if (c.neighbor_mode < 0 || c.neighbor_mode > 2) throw ArgError("rcp: neighbor_mode must be 0, 1 or 2");
+    if (c.neighbor_mode == 2 && c.exact_reductions)
+        throw ArgError("rcp: exact_reductions needs neighbour lists (neighbor_mode 0 or 1)");
     N_ = c.N;
     nd_ = c.ndim;
     opt_ = c;
@@ -151,11 +428,11 @@ Engine::Engine(const rcp_run_config& c)
         }
         D = std::move(Ds);
     }
-    double coeff = std::pow(M_PI, nd_ / 2.0) / std::tgamma(nd_ / 2.0 + 1.0);
+    const double coeff = std::pow(M_PI, nd_ / 2.0) / std::tgamma(nd_ / 2.0 + 1.0);
     double current_volume = 0.0;
-    for (std::size_t i = 0; i < N_; ++i)
-        current_volume += coeff * std::pow(D[i] / 2.0, nd_);
-    double box_volume = std::accumulate(box_.begin(), box_.end(), 1.0, std::multiplies<double>());
+    for (std::size_t i = 0; i < N_; ++i) current_volume += coeff * std::pow(D[i] / 2.0, nd_);
+    const double box_volume =
+        std::accumulate(box_.begin(), box_.end(), 1.0, std::multiplies<double>());
     phi_ = current_volume / box_volume / phi_modifier_;
 
     // step cap (RG:4713-4737), speed preset "quick"
@@ -202,105 +479,35 @@ Engine::Engine(const rcp_run_config& c)
     hip_check(hipStreamCreate(&stream_), "hipStreamCreate");
     if (opt_.time_kernels)
         for (auto& ev : ev_) hip_check(hipEventCreate(&ev), "hipEventCreate");
-    alloc_device();
-    hip_check(hipMemcpyAsync(x_[cur_], x.data(), N_ * nd_ * sizeof(double),
-                             hipMemcpyHostToDevice, stream_), "upload x");
-    hip_check(hipMemcpyAsync(D0_, D.data(), N_ * sizeof(double), hipMemcpyHostToDevice,
-                             stream_), "upload D0");
-    if (fixed_nonempty_)
-        hip_check(hipMemcpyAsync(fixed_, fixed.data(), N_, hipMemcpyHostToDevice, stream_),
-                  "upload fixed");
-    hip_check(hipMemsetAsync(m_, 0, N_ * nd_ * sizeof(double), stream_), "zero m");
-    hip_check(hipMemsetAsync(v_, 0, N_ * nd_ * sizeof(double), stream_), "zero v");
-    if (launch_iota_u64(origin_, N_, stream_)) throw HipFailure("iota launch");
+    st_ = new DeviceState((int)N_, (int)nd_, fixed_nonempty_, opt_.exact_reductions != 0,
+                          stream_);
+    st_->upload(x.data(), D.data(), fixed_nonempty_ ? fixed.data() : nullptr);
 
-    // initial reorder (RG:4990-4993) and first build + forces (RG:5011-5013)
-    if (opt_.reorder_interval > 0) reorder();
+    // initial reorder (RG:4990-4993) and the first force evaluation
+    // (RG:5011-5013: standard weights, mu)
+    if (opt_.reorder_interval > 0) {
+        st_->reference_reorder(geo_);
+        ++reorders_;
+    }
     kappa_D_ = 1.0;
+    kappa_prev_ = 1.0;
     build_lists();
-    const Scalars s0 = step_device(/*do_adam=*/false, /*alpha*/ 0.0, 1.0, 1.0, /*use_rk*/ 0);
-    finalize_scalars(s0, mu_, /*use_active*/ false);
+    const Scalars s0 = step_device(false, 0.0, 1.0, 1.0, 0, nullptr, false);
+    finalize_scalars(s0, mu_);
 }
 
 Engine::~Engine()
 {
-    if (stream_) (void)hipStreamSynchronize(stream_);
-    for (void* p : allocs_) dev_free(p);
-    if (h_result_) (void)hipHostFree(h_result_);
+    delete st_;
     for (auto& ev : ev_)
         if (ev) (void)hipEventDestroy(ev);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
-template <class T>
-T* Engine::dalloc(size_t n)
-{
-    T* p = static_cast<T*>(dev_alloc(n * sizeof(T)));
-    allocs_.push_back(p);
-    return p;
-}
-
-void Engine::dfree(void* p)
-{
-    auto it = std::find(allocs_.begin(), allocs_.end(), p);
-    if (it != allocs_.end()) {
-        dev_free(p);
-        allocs_.erase(it);
-    }
-}
-
-void Engine::alloc_device()
-{
-    const size_t Nd = N_ * nd_;
-    x_[0] = dalloc<double>(Nd);
-    x_[1] = dalloc<double>(Nd);
-    m_ = dalloc<double>(Nd);
-    v_ = dalloc<double>(Nd);
-    tmp_f64_ = dalloc<double>(Nd);
-    x_anchor_ = dalloc<double>(Nd);
-    D0_ = dalloc<double>(N_);
-    origin_ = dalloc<unsigned long long>(N_);
-    origin_tmp_ = dalloc<unsigned long long>(N_);
-    if (fixed_nonempty_) {
-        fixed_ = dalloc<int8_t>(N_);
-        fixed_tmp_ = dalloc<int8_t>(N_);
-    }
-    cell_of_ = dalloc<uint32_t>(N_);
-    slot_ = dalloc<uint32_t>(N_);
-    items_ = dalloc<uint32_t>(N_);
-    row_count_ = dalloc<uint32_t>(N_ + 1);
-    row_start_ = dalloc<uint32_t>(N_ + 1);
-    keys_ = dalloc<unsigned long long>(N_);
-    keys_alt_ = dalloc<unsigned long long>(N_);
-    vals_ = dalloc<uint32_t>(N_);
-    vals_alt_ = dalloc<uint32_t>(N_);
-    hist_ = dalloc<uint32_t>(sort_hist_entries(N_));
-    const size_t nblocks = (N_ + 255) / 256;
-    partials_ = dalloc<double>(nblocks * S_NUM + S_NUM);
-    result_ = dalloc<double>(S_NUM);
-    counter_ = dalloc<unsigned int>(1);
-    hip_check(hipMemsetAsync(counter_, 0, sizeof(unsigned int), stream_), "zero counter");
-    if (opt_.exact_reductions) {
-        F_ = dalloc<double>(Nd);
-        mag_ = dalloc<double>(N_);
-    }
-    hip_check(hipHostMalloc((void**)&h_result_, S_NUM * sizeof(double), hipHostMallocDefault),
-              "hipHostMalloc");
-    ensure_scan_tmp(sort_hist_entries(N_) + 8);
-}
-
-void Engine::ensure_scan_tmp(size_t n)
-{
-    const size_t bytes = scan_tmp_bytes(n);
-    if (bytes <= scan_tmp_bytes_) return;
-    if (scan_tmp_) dfree(scan_tmp_);
-    scan_tmp_ = dalloc<char>(bytes);
-    scan_tmp_bytes_ = bytes;
-}
-
 void Engine::recompute_d0_cache(const std::vector<double>& D0, const std::vector<int8_t>& fixed)
 {
-    // Dmin / S_D0 caches (RG:4808-4826); max per class for list sizing
+    // Dmin / S_D0 caches (RG:4808-4826) in reference order; largest free and
+    // frozen diameters size the cells and lists
     D0_min_ = D0[0];
     S_D0_ = 0.0;
     D0max_free_ = 0.0;
@@ -318,173 +525,70 @@ void Engine::recompute_d0_cache(const std::vector<double>& D0, const std::vector
     shell0_ = reference_shell(D0);
 }
 
-void Engine::download_d0(std::vector<double>& D0, std::vector<int8_t>& fixed)
+double Engine::dmax_now() const
 {
-    D0.resize(N_);
-    hip_check(hipMemcpyAsync(D0.data(), D0_, N_ * sizeof(double), hipMemcpyDeviceToHost,
-                             stream_), "download D0");
-    fixed.clear();
-    if (fixed_nonempty_) {
-        fixed.resize(N_);
-        hip_check(hipMemcpyAsync(fixed.data(), fixed_, N_, hipMemcpyDeviceToHost, stream_),
-                  "download fixed");
-    }
-    hip_check(hipStreamSynchronize(stream_), "sync");
+    return std::max(D0max_free_ * kappa_D_, D0max_fixed_);
 }
 
-// --------------------------------------------------------------- reorder
-
-void Engine::reorder()
+void Engine::begin_timer()
 {
-    // RG:4850-4948: Morton-21 (Ndim 3) or x-column order, stable by index;
-    // every per-particle array is permuted, then the lists are rebuilt.
-    double* x = x_[cur_];
-    if (launch_reorder_keys(x, (int)N_, (int)nd_, geo_.box, keys_, vals_, stream_))
-        throw HipFailure("reorder keys launch");
-    SortTmp t;
-    t.keys_alt = keys_alt_;
-    t.vals_alt = vals_alt_;
-    t.hist = hist_;
-    t.scan_tmp = scan_tmp_;
-    if (launch_radix_sort(keys_, vals_, N_, 64, t, stream_)) throw HipFailure("radix sort");
-    const uint32_t* perm = vals_;
-    auto permute = [&](double*& a, int w) {
-        if (launch_gather_f64(a, tmp_f64_, perm, (int)N_, w, stream_)) throw HipFailure("gather");
-        std::swap(a, tmp_f64_);
-    };
-    permute(x_[cur_], (int)nd_);
-    permute(m_, (int)nd_);
-    permute(v_, (int)nd_);
-    // D0 is N wide; reuse the N*ndim scratch
-    if (launch_gather_f64(D0_, tmp_f64_, perm, (int)N_, 1, stream_)) throw HipFailure("gather D0");
-    hip_check(hipMemcpyAsync(D0_, tmp_f64_, N_ * sizeof(double), hipMemcpyDeviceToDevice, stream_),
-              "copy D0");
-    if (launch_gather_u64(origin_, origin_tmp_, perm, (int)N_, stream_)) throw HipFailure("gather origin");
-    std::swap(origin_, origin_tmp_);
-    if (fixed_nonempty_) {
-        if (launch_gather_i8(fixed_, fixed_tmp_, perm, (int)N_, stream_)) throw HipFailure("gather fixed");
-        std::swap(fixed_, fixed_tmp_);
-    }
-    ++reorders_;
-    need_build_ = true;
+    if (opt_.time_kernels) hip_check(hipEventRecord(ev_[0], stream_), "event");
 }
 
-// ------------------------------------------------------------ neighbour lists
+void Engine::end_timer(double& acc_ms, uint64_t& n)
+{
+    if (!opt_.time_kernels) return;
+    hip_check(hipEventRecord(ev_[1], stream_), "event");
+    hip_check(hipEventSynchronize(ev_[1]), "event sync");
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "event time");
+    acc_ms += ms;
+    ++n;
+}
+
+// ------------------------------------------------------- neighbour lists
 
 void Engine::build_lists()
 {
+    // Rows at the reference's shell for the current sizes: every pair that
+    // can touch before the skin test in step_once fires is listed.
     const double kD = kappa_D_;
-    const double Dmax = std::max(D0max_free_ * kD, D0max_fixed_);
     t_list_ = shell0_ * kD;
-    if (!(t_list_ > 0.0)) t_list_ = 1e-3 * std::max(Dmax, 1e-300);
-    ListArgs la;
-    la.g = geo_;
-    la.x = x_[cur_];
-    la.D0 = D0_;
-    la.fixed = fixed_;
-    la.kappa = kD;
-    la.t_shell = t_list_;
-    la.half = 0;
-    configure_grid(Dmax + t_list_, la.grid);
-    if (opt_.time_kernels) hip_check(hipEventRecord(ev_[2], stream_), "event");
-    build_rows(la);
-    if (opt_.time_kernels) {
-        hip_check(hipEventRecord(ev_[3], stream_), "event");
-        hip_check(hipEventSynchronize(ev_[3]), "event sync");
-        float ms = 0.0f;
-        hip_check(hipEventElapsedTime(&ms, ev_[2], ev_[3]), "event time");
-        t_build_ms_ += ms;
-        ++t_build_n_;
-    }
-    hip_check(hipMemcpyAsync(x_anchor_, x_[cur_], N_ * nd_ * sizeof(double),
+    if (!(t_list_ > 0.0)) t_list_ = 1e-3 * std::max(dmax_now(), 1e-300);
+    begin_timer();
+    st_->rebin(geo_, dmax_now() + t_list_);
+    st_->build_rows(geo_, kD, t_list_, 0, 0);
+    hip_check(hipMemcpyAsync(st_->anchor, st_->x, N_ * nd_ * sizeof(double),
                              hipMemcpyDeviceToDevice, stream_), "anchor copy");
+    end_timer(t_build_ms_, t_build_n_);
     kappa_b_ = kD;
     fh_drift_ = 0.0;
     need_build_ = false;
+    list_age_ = 0;
     ++list_builds_;
+    list_entries_ += st_->nnz;
 }
 
-void Engine::configure_grid(double reach, CellGrid& grid)
-{
-    const double edge = reach * (1.0 + 1e-6);
-    long long total = 1;
-    for (std::size_t d = 0; d < nd_; ++d) {
-        long long n = (edge > 0.0) ? (long long)std::floor(box_[d] / edge) : 1;
-        if (n < 3) n = 1;
-        if (n > (1 << 20)) n = 1 << 20;
-        grid.nc[d] = (int)n;
-        total *= n;
-    }
-    // keep the table near the particle count (bigger cells stay correct)
-    const long long cap = std::max<long long>(64, 2 * (long long)N_);
-    while (total > cap) {
-        std::size_t dmax = 0;
-        for (std::size_t d = 1; d < nd_; ++d)
-            if (grid.nc[d] > grid.nc[dmax]) dmax = d;
-        int n = grid.nc[dmax] / 2;
-        if (n < 3) n = 1;
-        total = total / grid.nc[dmax] * n;
-        grid.nc[dmax] = n;
-    }
-    for (std::size_t d = 0; d < nd_; ++d) grid.cell_scale[d] = grid.nc[d] / box_[d];
-    grid.total = total;
-}
-
-void Engine::build_rows(ListArgs& la)
-{
-    const size_t total = (size_t)la.grid.total;
-    if (total + 1 > cell_cap_) {
-        if (cell_count_) { dfree(cell_count_); dfree(cell_start_); }
-        cell_cap_ = total + 1;
-        cell_count_ = dalloc<uint32_t>(cell_cap_);
-        cell_start_ = dalloc<uint32_t>(cell_cap_);
-    }
-    ensure_scan_tmp(std::max(total, N_) + 8);
-    la.cell_of = cell_of_;
-    la.slot = slot_;
-    la.cell_count = cell_count_;
-    la.cell_start = cell_start_;
-    la.items = items_;
-    la.row_count = row_count_;
-    la.row_start = row_start_;
-    if (launch_cells(la, scan_tmp_, scan_tmp_bytes_, stream_)) throw HipFailure("cells launch");
-    if (launch_rows_count(la, stream_)) throw HipFailure("rows count launch");
-    if (launch_scan_u32(row_count_, row_start_, N_, scan_tmp_, stream_))
-        throw HipFailure("rows scan launch");
-    uint32_t nnz = 0;
-    hip_check(hipMemcpyAsync(&nnz, row_start_ + N_, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                             stream_), "nnz download");
-    hip_check(hipStreamSynchronize(stream_), "sync nnz");
-    if (nnz + 1ull > nbr_cap_) {
-        if (nbr_) dfree(nbr_);
-        if (contrib_) dfree(contrib_);
-        nbr_cap_ = (size_t)(nnz * 1.25) + 1024;
-        nbr_ = dalloc<uint32_t>(nbr_cap_);
-        if (opt_.exact_reductions) contrib_ = dalloc<double>(nbr_cap_ * 4);
-    }
-    la.nbr = nbr_;
-    la.nbr_cap = nbr_cap_;
-    if (launch_rows_fill(la, stream_)) throw HipFailure("rows fill launch");
-    nnz_ = nnz;
-    list_entries_ += nnz;
-}
-
-// ------------------------------------------------------------ device step
+// --------------------------------------------------------------- device step
 
 Engine::Scalars Engine::step_device(bool do_adam, double alpha, double inv_b1, double inv_b2,
-                                    int use_rk, const double* rk)
+                                    int use_rk, const double* rk, bool direct)
 {
+    DeviceState& S = *st_;
     StepArgs a;
     a.g = geo_;
-    a.x_in = x_[cur_];
-    a.x_out = x_[1 - cur_];
-    a.m = m_;
-    a.v = v_;
-    a.D0 = D0_;
-    a.fixed = fixed_;
-    a.x_anchor = x_anchor_;
-    a.row_start = row_start_;
-    a.nbr = nbr_;
+    a.x_in = S.x;
+    a.x_out = S.x_alt;
+    a.m = S.m;
+    a.v = S.v;
+    a.D0 = S.D0;
+    a.fixed = S.fixed;
+    a.tag = S.tag;
+    a.x_anchor = direct ? nullptr : S.anchor;
+    a.row_start = S.row_start;
+    a.nbr = S.nbr;
+    a.grid = S.grid;
+    a.cell_start = S.cell_start;
     a.kappa = kappa_D_;
     a.use_rk = use_rk;
     a.rk = rk;
@@ -492,40 +596,38 @@ Engine::Scalars Engine::step_device(bool do_adam, double alpha, double inv_b1, d
     a.inv_b1 = inv_b1;
     a.inv_b2 = inv_b2;
     a.do_adam = do_adam ? 1 : 0;
-    a.partials = partials_;
-    a.counter = counter_;
-    a.result = result_;
-    if (opt_.exact_reductions) {
-        a.F_out = F_;
-        a.mag_out = mag_;
-        a.contrib = contrib_;
+    a.inv_tag = S.inv_tag;
+    a.partials = S.partials;
+    a.counter = S.counter;
+    a.result = S.result;
+    begin_timer();
+    if (direct) {
+        if (launch_step_direct(a, stream_)) throw HipFailure("direct step launch");
+    } else if (opt_.exact_reductions) {
+        a.mag_out = S.mag;
+        a.contrib = S.contrib;
+        if (launch_exact_step(a, S.ref_count, S.ref_start, S.ref_terms, S.scan_tmp, stream_))
+            throw HipFailure("exact step launch");
+    } else {
+        if (launch_step_list(a, stream_)) throw HipFailure("list step launch");
     }
-    if (opt_.time_kernels) hip_check(hipEventRecord(ev_[0], stream_), "event");
-    if (launch_step(a, stream_)) throw HipFailure("step launch");
-    if (opt_.time_kernels) hip_check(hipEventRecord(ev_[1], stream_), "event");
-    if (opt_.exact_reductions && launch_exact_reduce(a, stream_))
-        throw HipFailure("exact reduce launch");
-    hip_check(hipMemcpyAsync(h_result_, result_, S_NUM * sizeof(double), hipMemcpyDeviceToHost,
+    end_timer(t_step_ms_, t_step_n_);
+    hip_check(hipMemcpyAsync(S.h_result, S.result, S_NUM * sizeof(double), hipMemcpyDeviceToHost,
                              stream_), "scalars download");
     hip_check(hipStreamSynchronize(stream_), "step sync");
-    if (opt_.time_kernels) {
-        float ms = 0.0f;
-        hip_check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "event time");
-        t_step_ms_ += ms;
-        ++t_step_n_;
-    }
     Scalars s;
-    for (int k = 0; k < S_NUM; ++k) s.v[k] = h_result_[k];
+    for (int k = 0; k < S_NUM; ++k) s.v[k] = S.h_result[k];
+    if (do_adam) std::swap(S.x, S.x_alt);
+    if (s.v[S_OVERFLOW] != 0.0) ++overflow_steps_;
     return s;
 }
 
-void Engine::finalize_scalars(const Scalars& s, double mu_eff, bool use_active)
+void Engine::finalize_scalars(const Scalars& s, double mu_eff)
 {
     // RG:4191-4268 finalisation; count starts at 1
     const double count = 1.0 + s.v[S_CNT];
     U_ = std::pow(s.v[S_U] / count, 2);
     Lc_ = s.v[S_LC] / count;
-    (void)use_active;
     dkappa_ = s.v[S_DK] + mu_eff * s.v[S_SUMD];
     Fmean_ = -s.v[S_FM] / count;
     max_min_dist_ = s.v[S_MAXOV];
@@ -533,6 +635,7 @@ void Engine::finalize_scalars(const Scalars& s, double mu_eff, bool use_active)
     const double mean_mag = s.v[S_SMAG] / double((uint32_t)s.v[S_MCNT]);
     mean_force_ = mean_mag / std::sqrt(double(nd_));
     maxdisp2_ = s.v[S_MAXDISP2];
+    maxstep2_ = s.v[S_MAXSTEP2];
 }
 
 // ------------------------------------------------------------- schedule
@@ -654,10 +757,11 @@ double Engine::current_volume_now()
 {
     const double coeff = std::pow(M_PI, nd_ / 2.0) / std::tgamma(nd_ / 2.0 + 1.0);
     if (has_fixed_) {
-        // mixed scaling: sum over the current diameters (RG:5761-5766)
+        // mixed scaling: sum over the current diameters in reference order
+        // (RG:5761-5766)
         std::vector<double> D0;
         std::vector<int8_t> fx;
-        download_d0(D0, fx);
+        st_->d0_ref(D0, fx);
         double cv = 0.0;
         for (std::size_t i = 0; i < N_; ++i) {
             const double Di = fx[i] ? D0[i] : D0[i] * kappa_D_;
@@ -676,27 +780,43 @@ void Engine::lock_target_phi(double box_volume)
     // RG:5783-5832: rescale D to the target, freeze diameters
     std::vector<double> D0;
     std::vector<int8_t> fx;
-    download_d0(D0, fx);
+    st_->d0_ref(D0, fx);
     std::vector<double> D(N_);
     for (std::size_t i = 0; i < N_; ++i)
         D[i] = (!fx.empty() && fx[i]) ? D0[i] : D0[i] * kappa_D_;
     std::vector<double> Ds;
     scale_diameters(D, opt_.target_phi * phi_modifier_, box_, nd_, opt_.fix_height != 0, Ds);
-    hip_check(hipMemcpyAsync(D0_, Ds.data(), N_ * sizeof(double), hipMemcpyHostToDevice, stream_),
-              "upload D0");
+    // D0 = D: every particle now carries its own diameter in D0; the
+    // cascade mask keeps selecting D0 for frozen particles, which equals D
+    st_->set_d0_ref(Ds, fx);
     kappa_ = 1.0;
     kappa_D_ = 1.0;
     mu_ = 0.0;
     runtime_fix_ = true;
     target_locked_ = true;
-    // D0 = D: every particle now carries its own diameter in D0; the
-    // cascade mask keeps selecting D0 for frozen particles, which equals D
     recompute_d0_cache(Ds, fx);
     const double coeff = std::pow(M_PI, nd_ / 2.0) / std::tgamma(nd_ / 2.0 + 1.0);
-    const double cv = coeff * S_D0_;
-    phi_ = cv / box_volume;
+    phi_ = coeff * S_D0_ / box_volume;
     need_build_ = true;
-    hip_check(hipStreamSynchronize(stream_), "sync");
+}
+
+void Engine::release_cascade()
+{
+    // RG:5910-5921: D0 of frozen particles re-anchored so D = D0*kappa stays
+    // continuous, then the mask is cleared
+    std::vector<double> D0;
+    std::vector<int8_t> fx;
+    st_->d0_ref(D0, fx);
+    for (std::size_t i = 0; i < N_; ++i) {
+        if (fx[i]) D0[i] = D0[i] / kappa_D_;
+        fx[i] = 0;
+    }
+    st_->set_d0_ref(D0, fx);
+    D0max_free_ = std::max(D0max_free_, D0max_fixed_ / kappa_D_);
+    D0max_fixed_ = 0.0;
+    shell0_ = reference_shell(D0);
+    cascade_released_ = true;
+    need_build_ = true;
 }
 
 bool Engine::step_once()
@@ -708,8 +828,13 @@ bool Engine::step_once()
         done_ = true;
         return false;
     }
-    // periodic spatial reorder (RG:5661-5663)
-    if (opt_.reorder_interval > 0 && step > 1 && (step % opt_.reorder_interval == 0)) reorder();
+    // periodic spatial reorder (RG:5661-5663): a new reference order; the
+    // rows are ordered by tag, so they are rebuilt
+    if (opt_.reorder_interval > 0 && step > 1 && (step % opt_.reorder_interval == 0)) {
+        st_->reference_reorder(geo_);
+        ++reorders_;
+        need_build_ = true;
+    }
     schedule_alpha(step);
 
     // growth (RG:5723-5834): D = D0 * kappa evaluated inside the kernels
@@ -722,7 +847,7 @@ bool Engine::step_once()
         box_[nd_ - 1] = box_[nd_ - 1] * kappa_;
         geo_.box[nd_ - 1] = box_[nd_ - 1];
         geo_.inv_box[nd_ - 1] = 1.0 / box_[nd_ - 1];
-        if (launch_scale_last_dim(x_[cur_], (int)N_, (int)nd_, kappa_, stream_))
+        if (launch_scale_last_dim(st_->x, (int)N_, (int)nd_, kappa_, stream_))
             throw HipFailure("scale launch");
         fh_drift_ += std::fabs(box_[nd_ - 1] - L_old);
     }
@@ -746,11 +871,47 @@ bool Engine::step_once()
         }
     }
 
-    // neighbour-list maintenance: rebuild when the skin could be crossed
-    {
+    // Neighbour handling.  List mode reuses the rows until a particle could
+    // have crossed half the skin (diameter growth and box stretch included).
+    // When rows live only a couple of steps (violent early phase) the direct
+    // cell scan replaces them; it hands back once the per-step motion lets a
+    // list live long enough to pay for its build.
+    const double growth_step = D0max_free_ * std::fabs(kappa_D_ - kappa_prev_);
+    kappa_prev_ = kappa_D_;
+    if (opt_.neighbor_mode == 2) {
+        mode_ = Mode::Direct;
+    } else if (opt_.neighbor_mode == 0 && !opt_.exact_reductions && mode_ == Mode::Direct) {
+        const double per_step = 2.0 * std::sqrt(maxstep2_) + growth_step;
+        const double t_est = shell0_ * kappa_D_;
+        if (direct_run_ >= 4 && per_step * 12.0 < t_est) {
+            mode_ = Mode::List;
+            need_build_ = true;
+        }
+    }
+    if (mode_ == Mode::List) {
         const double growth = std::max(0.0, D0max_free_ * (kappa_D_ - kappa_b_));
         const double reach = 2.0 * std::sqrt(maxdisp2_) + growth + 2.0 * fh_drift_;
-        if (need_build_ || !(reach < t_list_ * (1.0 - 1e-6))) build_lists();
+        const bool stale = !(reach < t_list_ * (1.0 - 1e-6));
+        if (need_build_ || stale) {
+            if (opt_.neighbor_mode == 0 && !opt_.exact_reductions && !need_build_ &&
+                list_age_ < 3) {
+                mode_ = Mode::Direct;
+                direct_run_ = 0;
+            } else {
+                build_lists();
+            }
+        }
+    }
+    const bool direct = (mode_ == Mode::Direct);
+    if (direct) {
+        begin_timer();
+        st_->rebin(geo_, dmax_now());
+        end_timer(t_build_ms_, t_build_n_);
+        need_build_ = true;     // rows are gone with the re-binning
+        ++direct_run_;
+        ++direct_steps_total_;
+    } else {
+        ++list_age_;
     }
 
     // forces (RG:6211-6237) + fused ADAM (RG:6409-6521)
@@ -759,16 +920,16 @@ bool Engine::step_once()
     const double inv_b1 = 1.0 / (1.0 - std::pow(0.9, t_));
     const double inv_b2 = 1.0 / (1.0 - std::pow(0.999, t_));
     // In the release step the reference clears the mask before the force call
-    // while D still holds the frozen sizes: all particles count as active.
+    // while D still holds the frozen sizes: every particle counts as active.
     const double* rk = nullptr;
     if (release_now) {
-        if (!rk_tmp_) rk_tmp_ = dalloc<double>(N_);
-        if (launch_rk_active(D0_, fixed_, kappa_D_, rk_tmp_, (int)N_, stream_))
+        if (!st_->rk) st_->rk = st_->alloc<double>(N_);
+        if (launch_rk_active(st_->D0, st_->fixed, kappa_D_, st_->rk, (int)N_, stream_))
             throw HipFailure("rk launch");
-        rk = rk_tmp_;
+        rk = st_->rk;
     }
-    const Scalars s = step_device(true, alpha_, inv_b1, inv_b2, has_fixed_ ? 1 : 0, rk);
-    finalize_scalars(s, mu_eff, has_fixed_);
+    const Scalars s = step_device(true, alpha_, inv_b1, inv_b2, has_fixed_ ? 1 : 0, rk, direct);
+    finalize_scalars(s, mu_eff);
     force_magnitude_ = mean_force_ / Fmean_;
     F_h_[step - 1] = force_magnitude_;
     ov_h_[step - 1] = max_min_dist_;
@@ -783,53 +944,17 @@ bool Engine::step_once()
     const double v_hat = v_kappa_ / (1.0 - std::pow(0.999, t_));
     if (!runtime_fix_) kappa_ = kappa_ - alpha_ * m_hat / (std::sqrt(v_hat) + 1e-8);
 
-    cur_ = 1 - cur_;
     if (release_now) release_cascade();
     U_h_[step - 1] = U_;
     hist_filled_ = step;
     return true;
 }
 
-void Engine::release_cascade()
-{
-    // RG:5910-5921: D0 of frozen particles re-anchored so D = D0*kappa stays
-    // continuous, then the mask is cleared
-    std::vector<double> D0;
-    std::vector<int8_t> fx;
-    download_d0(D0, fx);
-    for (std::size_t i = 0; i < N_; ++i) {
-        if (fx[i]) D0[i] = D0[i] / kappa_D_;
-        fx[i] = 0;
-    }
-    hip_check(hipMemcpyAsync(D0_, D0.data(), N_ * sizeof(double), hipMemcpyHostToDevice, stream_),
-              "upload D0");
-    hip_check(hipMemcpyAsync(fixed_, fx.data(), N_, hipMemcpyHostToDevice, stream_),
-              "upload fixed");
-    D0max_free_ = std::max(D0max_free_, D0max_fixed_ / kappa_D_);
-    D0max_fixed_ = 0.0;
-    shell0_ = reference_shell(D0);
-    cascade_released_ = true;
-    need_build_ = true;
-    hip_check(hipStreamSynchronize(stream_), "sync");
-}
+void Engine::current_diameters(std::vector<double>& D) { st_->diameters_ref(kappa_D_, D); }
 
-void Engine::current_diameters(std::vector<double>& D)
-{
-    std::vector<double> D0;
-    std::vector<int8_t> fx;
-    download_d0(D0, fx);
-    D.resize(N_);
-    for (std::size_t i = 0; i < N_; ++i)
-        D[i] = (!fx.empty() && fx[i]) ? D0[i] : D0[i] * kappa_D_;
-}
+void Engine::current_positions(std::vector<double>& x) { st_->to_ref(st_->x, (int)nd_, x); }
 
-void Engine::origin(std::vector<uint64_t>& o)
-{
-    o.resize(N_);
-    hip_check(hipMemcpyAsync(o.data(), origin_, N_ * sizeof(uint64_t), hipMemcpyDeviceToHost,
-                             stream_), "download origin");
-    hip_check(hipStreamSynchronize(stream_), "sync");
-}
+void Engine::origin(std::vector<uint64_t>& o) { st_->origin_ref(o); }
 
 void Engine::histories(double* phi, double* F, double* U, double* ov, double* mu, double* al,
                        int8_t* fl) const
@@ -845,14 +970,6 @@ void Engine::histories(double* phi, double* F, double* U, double* ov, double* mu
     cp(mu, mu_h_);
     cp(al, al_h_);
     if (fl && n) std::memcpy(fl, fl_h_.data(), n);
-}
-
-void Engine::current_positions(std::vector<double>& x)
-{
-    x.resize(N_ * nd_);
-    hip_check(hipMemcpyAsync(x.data(), x_[cur_], N_ * nd_ * sizeof(double),
-                             hipMemcpyDeviceToHost, stream_), "download x");
-    hip_check(hipStreamSynchronize(stream_), "sync");
 }
 
 int Engine::record(std::size_t step, bool capture, bool final_sample, rcp_observer_fn fn,
@@ -876,7 +993,7 @@ int Engine::record(std::size_t step, bool capture, bool final_sample, rcp_observ
         o.force_magnitude = t.force;
         o.energy = t.energy;
         o.max_min_dist = t.max_min_dist;
-        const double* pos = capture ? (keep ? trace_.back().positions.data() : nullptr) : nullptr;
+        const double* pos = capture ? trace_.back().positions.data() : nullptr;
         if (fn(user, &o, pos)) return 1;
     }
     return 0;
@@ -890,7 +1007,8 @@ int Engine::run(std::size_t nsteps, std::size_t progress_interval, bool capture,
         const bool capture_initial = capture && trajectory_interval > 0;
         const bool report_initial = fn && progress_interval > 0;
         if (capture_initial || report_initial) {
-            // initial sample: the trace keeps it only when capturing
+            // the initial sample (RG:5045-5052): kept in the trace only when
+            // capturing positions
             TraceSample t;
             t.step = 0;
             t.phi = phi_ / phi_modifier_;

@@ -36,12 +36,81 @@ double scale_diameters(const std::vector<double>& D, double phi_target,
                        const std::vector<double>& box, std::size_t ndim, bool fix_height,
                        std::vector<double>& out);
 double reference_shell(std::vector<double> D);
+// Cell grid with cells at least `reach` wide on every axis (fewer than three
+// cells on an axis collapses it to one), total kept near `cap` cells.
+void make_grid(const double* box, int ndim, double reach, long long cap, CellGrid& grid);
 
 struct TraceSample {
     std::size_t step = 0;
     double phi = 0, force = 0, energy = 0, max_min_dist = 0;
     std::vector<double> positions, diameters;
     bool has_positions = false;
+};
+
+// Device arrays of one packing in working order, with the binning and list
+// scratch.  Shared by the engine and the one-shot kernel-level calls.
+class DeviceState {
+public:
+    DeviceState(int N, int ndim, bool with_fixed, bool exact, hipStream_t s);
+    ~DeviceState();
+    DeviceState(const DeviceState&) = delete;
+    DeviceState& operator=(const DeviceState&) = delete;
+
+    // upload reference-ordered positions / diameters / mask (tags = identity)
+    void upload(const double* x, const double* D0, const int8_t* fixed);
+    // re-bin into cell order for cells at least `reach` wide
+    void rebin(const Geometry& g, double reach);
+    // rows of the current binning: rc = (Di + Dj)*0.5 + t_shell
+    void build_rows(const Geometry& g, double kappa, double t_shell, int half, int sortwalk_rc);
+    // reference reorder of the tags
+    void reference_reorder(const Geometry& g);
+    void to_ref(const double* src_w, int width, std::vector<double>& out);
+    void d0_ref(std::vector<double>& D0, std::vector<int8_t>& fixed);
+    void set_d0_ref(const std::vector<double>& D0, const std::vector<int8_t>& fixed);
+    void origin_ref(std::vector<uint64_t>& o);
+    void diameters_ref(double kappa, std::vector<double>& D);
+    void sync();
+
+    int N, nd;
+    bool has_fixed_array, exact;
+    hipStream_t s;
+    CellGrid grid;
+    uint32_t nnz = 0;
+    bool binned = false;
+    // state (working order); *_alt are re-binning destinations
+    double *x = nullptr, *x_alt = nullptr, *m = nullptr, *m_alt = nullptr, *v = nullptr,
+           *v_alt = nullptr, *D0 = nullptr, *D0_alt = nullptr, *anchor = nullptr,
+           *tmp = nullptr, *tmp2 = nullptr, *rk = nullptr;
+    uint32_t *tag = nullptr, *tag_alt = nullptr, *inv_tag = nullptr, *tag_tmp = nullptr;
+    unsigned long long *origin = nullptr, *origin_alt = nullptr, *u64tmp = nullptr;
+    int8_t *fixed = nullptr, *fixed_alt = nullptr, *i8tmp = nullptr;
+    // binning / rows
+    uint32_t *cell_of = nullptr, *slot = nullptr, *perm = nullptr, *cell_count = nullptr,
+             *cell_start = nullptr, *row_count = nullptr, *row_start = nullptr, *nbr = nullptr;
+    size_t cell_cap = 0, nbr_cap = 0;
+    // exact reductions
+    double *contrib = nullptr, *mag = nullptr, *ref_terms = nullptr;
+    uint32_t *ref_count = nullptr, *ref_start = nullptr;
+    // reductions
+    double *partials = nullptr, *result = nullptr, *h_result = nullptr;
+    unsigned int* counter = nullptr;
+    // sorting
+    SortTmp sort;
+    char* scan_tmp = nullptr;
+    size_t scan_bytes = 0;
+
+    template <class T>
+    T* alloc(size_t n)
+    {
+        T* p = static_cast<T*>(dev_alloc(n * sizeof(T)));
+        allocs_.push_back(p);
+        return p;
+    }
+    void release(void* p);
+    void ensure_scan(size_t n);
+
+private:
+    std::vector<void*> allocs_;
 };
 
 class Engine {
@@ -76,6 +145,8 @@ public:
     uint64_t list_builds() const { return list_builds_; }
     uint64_t list_entries() const { return list_entries_; }
     uint64_t reorders() const { return reorders_; }
+    uint64_t direct_steps() const { return direct_steps_total_; }
+    uint64_t overflow_steps() const { return overflow_steps_; }
     void timing(double& step_ms, uint64_t& steps, double& build_ms, uint64_t& builds) const
     {
         step_ms = t_step_ms_;
@@ -87,20 +158,14 @@ public:
 private:
     struct Scalars { double v[S_NUM]; };
     enum class Osc { Inactive, Active, Final, Done };
+    enum class Mode { List, Direct };
 
-    template <class T> T* dalloc(size_t n);
-    void dfree(void* p);
-    void alloc_device();
-    void ensure_scan_tmp(size_t n);
     void recompute_d0_cache(const std::vector<double>& D0, const std::vector<int8_t>& fixed);
-    void download_d0(std::vector<double>& D0, std::vector<int8_t>& fixed);
-    void reorder();
+    double dmax_now() const;
     void build_lists();
-    void configure_grid(double reach, CellGrid& grid);
-    void build_rows(ListArgs& la);
     Scalars step_device(bool do_adam, double alpha, double inv_b1, double inv_b2, int use_rk,
-                        const double* rk = nullptr);
-    void finalize_scalars(const Scalars& s, double mu_eff, bool use_active);
+                        const double* rk, bool direct);
+    void finalize_scalars(const Scalars& s, double mu_eff);
     bool schedule_top(std::size_t step);
     void schedule_alpha(std::size_t step);
     double current_volume_now();
@@ -109,6 +174,8 @@ private:
     bool step_once();
     int record(std::size_t step, bool capture, bool final_sample, rcp_observer_fn fn,
                void* user);
+    void begin_timer();
+    void end_timer(double& acc_ms, uint64_t& n);
 
     // configuration
     std::size_t N_ = 0, nd_ = 0;
@@ -119,7 +186,7 @@ private:
     double phi_modifier_ = 1.0;
     bool has_fixed_ = false, fixed_nonempty_ = false, cascade_released_ = false;
 
-    // schedule state (RG:4655-4800 names)
+    // schedule state
     std::size_t step_cap_ = 0, rung1_deadline_ = 0;
     double phi_ = 0.0, t_ = 0.0;
     double alpha_ = 0.0, alpha_max_ = 0.0, mu_ = 0.0;
@@ -138,7 +205,7 @@ private:
     double D0_min_ = 0.0, S_D0_ = 0.0, D0max_free_ = 0.0, D0max_fixed_ = 0.0, shell0_ = 0.0;
     // force-evaluation outputs
     double U_ = 0.0, Lc_ = 0.0, Fmean_ = 0.0, max_min_dist_ = 0.0, mean_force_ = 0.0;
-    double force_magnitude_ = 0.0, maxdisp2_ = 0.0;
+    double force_magnitude_ = 0.0, maxdisp2_ = 0.0, maxstep2_ = 0.0;
     // histories
     std::vector<double> phi_h_, F_h_, U_h_, ov_h_, mu_h_, al_h_;
     std::vector<int8_t> fl_h_;
@@ -146,36 +213,22 @@ private:
     bool done_ = false, initial_recorded_ = false;
     std::vector<TraceSample> trace_;
 
-    // neighbour lists
-    double t_list_ = 0.0, kappa_b_ = 1.0, fh_drift_ = 0.0;
+    // neighbour handling
+    Mode mode_ = Mode::List;
+    double t_list_ = 0.0, kappa_b_ = 1.0, fh_drift_ = 0.0, kappa_prev_ = 1.0;
     bool need_build_ = true;
-    uint64_t list_builds_ = 0, list_entries_ = 0, reorders_ = 0;
-    size_t nnz_ = 0;
+    std::size_t list_age_ = 0, direct_run_ = 0;
+    uint64_t list_builds_ = 0, list_entries_ = 0, reorders_ = 0, direct_steps_total_ = 0,
+             overflow_steps_ = 0;
 
-    // kernel timing (HIP events on stream_)
-    hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
+    // timing (HIP events on stream_)
+    hipEvent_t ev_[2] = {nullptr, nullptr};
     double t_step_ms_ = 0.0, t_build_ms_ = 0.0;
     uint64_t t_step_n_ = 0, t_build_n_ = 0;
 
-    // device
     Geometry geo_;
     hipStream_t stream_ = nullptr;
-    std::vector<void*> allocs_;
-    double* x_[2] = {nullptr, nullptr};
-    int cur_ = 0;
-    double *m_ = nullptr, *v_ = nullptr, *tmp_f64_ = nullptr, *x_anchor_ = nullptr,
-           *D0_ = nullptr, *F_ = nullptr, *mag_ = nullptr, *contrib_ = nullptr,
-           *partials_ = nullptr, *result_ = nullptr, *rk_tmp_ = nullptr;
-    double* h_result_ = nullptr;
-    int8_t *fixed_ = nullptr, *fixed_tmp_ = nullptr;
-    unsigned long long *origin_ = nullptr, *origin_tmp_ = nullptr, *keys_ = nullptr,
-                       *keys_alt_ = nullptr;
-    uint32_t *cell_of_ = nullptr, *slot_ = nullptr, *items_ = nullptr, *row_count_ = nullptr,
-             *row_start_ = nullptr, *nbr_ = nullptr, *cell_count_ = nullptr,
-             *cell_start_ = nullptr, *vals_ = nullptr, *vals_alt_ = nullptr, *hist_ = nullptr;
-    unsigned int* counter_ = nullptr;
-    char* scan_tmp_ = nullptr;
-    size_t scan_tmp_bytes_ = 0, cell_cap_ = 0, nbr_cap_ = 0;
+    DeviceState* st_ = nullptr;
 };
 
 }  // namespace rcp

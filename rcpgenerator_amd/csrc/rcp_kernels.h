@@ -1,6 +1,13 @@
 // Launch interface between the host engine (compiled with g++) and the HIP
-// kernels (compiled with hipcc for gfx950).  Everything here is plain C++
-// over raw device pointers; no HIP types other than hipStream_t.
+// kernels (compiled with hipcc for gfx950).  Plain C++ over raw device
+// pointers; no HIP types other than hipStream_t.
+//
+// Layout.  Per-particle state lives in a *working order* w that is the
+// cell order of the last re-binning (spatially coherent: a wavefront's
+// particles and their neighbours share cache lines).  Each particle carries
+// tag[w] = its index in the reference engine's order; partner sums run in
+// ascending tag order and every output is scattered back through the tags,
+// so the arithmetic is that of the reference's index order.
 #pragma once
 
 #include <cstddef>
@@ -12,20 +19,21 @@ namespace rcp {
 
 constexpr int kMaxDim = 8;
 
-// Scalar slots produced by one fused step (sums over the particles /
-// pairs each block owns; the last block folds all block partials in block
-// order, so the totals are deterministic run to run).
+// Scalar slots of one fused step (block partials folded by the last block
+// in block order: deterministic run to run).
 enum StepScalar : int {
-    S_U = 0,      // sum of pair overlaps (j > i) + wall / curved overlap terms
-    S_LC,         // sum of r_ij over contacts
-    S_DK,         // sum r (dist - r) terms (pairs + walls + curved), without mu
+    S_U = 0,      // pair overlaps (tag_j > tag_i) + wall / curved terms
+    S_LC,         // sum of r_ij over pair contacts
+    S_DK,         // r (dist - r) terms (pairs + walls + curved), without mu
     S_FM,         // sum of F_mag over contacts
     S_SUMD,       // sum of the (active) diameters
     S_SMAG,       // sum |F_i| over |F_i|^2 > 1e-16
-    S_CNT,        // number of contacts (pairs + walls)
-    S_MCNT,       // number of particles with |F_i|^2 > 1e-16
+    S_CNT,        // contacts (pairs + walls)
+    S_MCNT,       // particles with |F_i|^2 > 1e-16
     S_MAXOV,      // max pair overlap
     S_MAXDISP2,   // max |x_new - x_anchor|^2 (minimum image)
+    S_MAXSTEP2,   // max |x_new - x_old|^2 (minimum image)
+    S_OVERFLOW,   // particles whose contact buffer overflowed (direct mode)
     S_NUM
 };
 
@@ -38,63 +46,103 @@ struct Geometry {
     int curved_T = 0;   // -walls[0] when walls[0] < 0, else 0
 };
 
+struct CellGrid {
+    int nc[kMaxDim] = {};             // cells per axis (1 = axis not split)
+    double cell_scale[kMaxDim] = {};  // nc / L
+    long long total = 1;
+};
+
+// Fused step over the working order.
 struct StepArgs {
     Geometry g;
-    const double* x_in = nullptr;    // N*ndim, positions before the step
-    double* x_out = nullptr;         // N*ndim, positions after ADAM + wrap
+    const double* x_in = nullptr;    // N*ndim positions before the step
+    double* x_out = nullptr;         // N*ndim positions after ADAM + wrap
     double* m = nullptr;             // N*ndim
     double* v = nullptr;             // N*ndim
     const double* D0 = nullptr;      // N
     const int8_t* fixed = nullptr;   // N or null
+    const uint32_t* tag = nullptr;   // N reference index of each particle
     const double* x_anchor = nullptr;// N*ndim positions at the last list build
-    const uint32_t* row_start = nullptr; // N+1
-    const uint32_t* nbr = nullptr;   // full rows, ascending partner index
+    // list mode: rows of working positions, ascending partner tag
+    const uint32_t* row_start = nullptr;
+    const uint32_t* nbr = nullptr;
+    // direct mode: cell ranges of the current working order
+    CellGrid grid;
+    const uint32_t* cell_start = nullptr;
     double kappa = 1.0;              // D_i = D0_i * kappa (fixed: D0_i)
-    int use_rk = 0;                  // cascade dkappa weights: rk_i = 0 if fixed, else D_i/2
-    const double* rk = nullptr;      // explicit rk_i (overrides the computed weights)
+    int use_rk = 0;                  // cascade weights: rk_i = 0 if fixed, else D_i/2
+    const double* rk = nullptr;      // explicit rk (working order) overriding the above
     double alpha = 0.0, inv_b1 = 1.0, inv_b2 = 1.0;
     int do_adam = 1;                 // 0: forces only (x_out untouched)
-    double* F_out = nullptr;         // optional N*ndim
+    double* F_out = nullptr;         // optional N*ndim (working order)
     double* mag_out = nullptr;       // optional N: sqrt(|F|^2) or -1 (exact mode)
     double* contrib = nullptr;       // optional 4 doubles per row slot (exact mode)
+    const uint32_t* inv_tag = nullptr; // N: working position of reference index
     double* partials = nullptr;      // gridDim * S_NUM
     unsigned int* counter = nullptr; // last-block ticket
     double* result = nullptr;        // S_NUM totals
 };
 
-// One fused step: forces + walls + reductions + (optional) ADAM/wrap.
-int launch_step(const StepArgs& a, hipStream_t s);
-// Sequential (reference 1-thread order) scalar reduction for exact mode.
-int launch_exact_reduce(const StepArgs& a, hipStream_t s);
+// blocks of the widest step launch over n particles (partials: +1 rows)
+size_t step_partial_blocks(size_t n);
+int launch_step_list(const StepArgs& a, hipStream_t s);
+int launch_step_direct(const StepArgs& a, hipStream_t s);
+// List step whose scalar terms are then folded in the reference's
+// one-thread order (a.contrib: 4 per row slot, a.mag_out: N, a.inv_tag).
+// ref_count / ref_start: N+1; ref_terms: 4 per recorded pair term.
+int launch_exact_step(const StepArgs& a, uint32_t* ref_count, uint32_t* ref_start,
+                      double* ref_terms, void* scan_tmp, hipStream_t s);
+
+// ---------------------------------------------------------------- binning
+// Re-binning to a new working order: old-order arrays are gathered into
+// cell order (within a cell by ascending tag), inv_tag is rebuilt.
+struct BinArgs {
+    Geometry g;
+    CellGrid grid;
+    // old working order
+    const double* x = nullptr;
+    const double* m = nullptr;
+    const double* v = nullptr;
+    const double* D0 = nullptr;
+    const uint32_t* tag = nullptr;
+    const unsigned long long* origin = nullptr;
+    const int8_t* fixed = nullptr;
+    // new working order
+    double* x_o = nullptr;
+    double* m_o = nullptr;
+    double* v_o = nullptr;
+    double* D0_o = nullptr;
+    uint32_t* tag_o = nullptr;
+    unsigned long long* origin_o = nullptr;
+    int8_t* fixed_o = nullptr;
+    uint32_t* inv_tag = nullptr;
+    // scratch
+    uint32_t* cell_of = nullptr;       // N
+    uint32_t* slot = nullptr;          // N
+    uint32_t* cell_count = nullptr;    // total + 1
+    uint32_t* cell_start = nullptr;    // total + 1
+    uint32_t* perm = nullptr;          // N: old position of new position q
+};
+int launch_rebin(BinArgs& a, void* scan_tmp, hipStream_t s);
 
 // ---------------------------------------------------------------- lists
-struct CellGrid {
-    int nc[kMaxDim] = {};     // cells per axis (1 = axis not split)
-    double cell_scale[kMaxDim] = {};  // nc / L
-    long long total = 1;
-};
-
 struct ListArgs {
     Geometry g;
-    const double* x = nullptr;
+    CellGrid grid;
+    const double* x = nullptr;         // working order (cell order)
     const double* D0 = nullptr;
     const int8_t* fixed = nullptr;
+    const uint32_t* tag = nullptr;
+    const uint32_t* cell_start = nullptr;
     double kappa = 1.0;
-    double t_shell = 0.0;          // list cutoff rc = (Di+Dj)*0.5 + t_shell
-    int half = 0;                  // 1: keep only j > i (pair-set API)
-    CellGrid grid;
-    uint32_t* cell_of = nullptr;   // N
-    uint32_t* slot = nullptr;      // N
-    uint32_t* cell_count = nullptr;// total+1
-    uint32_t* cell_start = nullptr;// total+1
-    uint32_t* items = nullptr;     // N
-    uint32_t* row_count = nullptr; // N+1
-    uint32_t* row_start = nullptr; // N+1
-    uint32_t* nbr = nullptr;       // capacity nbr_cap
+    double t_shell = 0.0;              // rc = (Di + Dj) * 0.5 + t_shell
+    int half = 0;                      // 1: partners with tag_j > tag_i only
+    int sortwalk_rc = 0;               // 1: rc = (Di*0.5 + t) + (double)(float)(Dj*0.5)
+    uint32_t* row_count = nullptr;     // N + 1
+    uint32_t* row_start = nullptr;     // N + 1
+    uint32_t* nbr = nullptr;
     unsigned long long nbr_cap = 0;
 };
-
-int launch_cells(ListArgs& a, void* scan_tmp, size_t scan_tmp_bytes, hipStream_t s);
 int launch_rows_count(ListArgs& a, hipStream_t s);
 int launch_rows_fill(ListArgs& a, hipStream_t s);
 
@@ -102,45 +150,62 @@ int launch_rows_fill(ListArgs& a, hipStream_t s);
 size_t scan_tmp_bytes(size_t n);
 int launch_scan_u32(const uint32_t* in, uint32_t* out, size_t n, void* tmp, hipStream_t s);
 
-// ---------------------------------------------------------------- reorder
+// ------------------------------------------------------- reference order
 struct SortTmp {
+    unsigned long long* keys = nullptr;
     unsigned long long* keys_alt = nullptr;
+    uint32_t* vals = nullptr;
     uint32_t* vals_alt = nullptr;
-    uint32_t* hist = nullptr;      // 256 * nblocks + 1
+    uint32_t* hist = nullptr;          // sort_hist_entries(n)
     void* scan_tmp = nullptr;
 };
 size_t sort_hist_entries(size_t n);
-// keys: Morton-21 code (ndim == 3) or the ordered bits of x[i*ndim] (other
-// ndim); vals = identity; then a stable LSD radix sort.
-int launch_reorder_keys(const double* x, int N, int ndim, const double* box,
-                        unsigned long long* keys, uint32_t* vals, hipStream_t s);
 int launch_radix_sort(unsigned long long* keys, uint32_t* vals, size_t n, int key_bits,
                       SortTmp& t, hipStream_t s);
-// dst[k*w + d] = src[perm[k]*w + d]
-int launch_gather_f64(const double* src, double* dst, const uint32_t* perm, int N, int w,
-                      hipStream_t s);
-int launch_gather_u64(const unsigned long long* src, unsigned long long* dst,
-                      const uint32_t* perm, int N, hipStream_t s);
-int launch_gather_i8(const int8_t* src, int8_t* dst, const uint32_t* perm, int N,
-                     hipStream_t s);
+// The reference's spatial reorder (Morton-21 for ndim 3, x column otherwise,
+// stable in the current reference order) applied to the tags only.
+// tag_tmp: N scratch.
+int launch_reference_reorder(const Geometry& g, const double* x, uint32_t* tag,
+                             uint32_t* inv_tag, uint32_t* tag_tmp, SortTmp& t, hipStream_t s);
+// Permutation of the reference reorder on reference-ordered positions
+// (kernel-level API): perm[k] = old index at slot k.
+int launch_reorder_perm(const Geometry& g, const double* x_ref, uint32_t* perm, SortTmp& t,
+                        hipStream_t s);
 
-// ---------------------------------------------------------------- small
+// --------------------------------------------------------- scatters/misc
+// dst[tag[w]*width + d] = src[w*width + d]   (working -> reference order)
+int launch_to_ref_f64(const double* src, double* dst, const uint32_t* tag, int N, int width,
+                      hipStream_t s);
+int launch_to_ref_u64(const unsigned long long* src, unsigned long long* dst,
+                      const uint32_t* tag, int N, hipStream_t s);
+int launch_to_ref_i8(const int8_t* src, int8_t* dst, const uint32_t* tag, int N, hipStream_t s);
+// dst[w*width + d] = src[tag[w]*width + d]   (reference -> working order)
+int launch_from_ref_f64(const double* src, double* dst, const uint32_t* tag, int N, int width,
+                        hipStream_t s);
+int launch_from_ref_i8(const int8_t* src, int8_t* dst, const uint32_t* tag, int N,
+                       hipStream_t s);
+// Rows of a list in reference order: for the particle with tag k, the tags
+// of its partners (ascending).  ref_count: N+1; then ref_nbr.
+int launch_rows_to_ref_count(const uint32_t* row_start, const uint32_t* tag, uint32_t* ref_count,
+                             int N, hipStream_t s);
+int launch_rows_to_ref_fill(const uint32_t* row_start, const uint32_t* nbr, const uint32_t* tag,
+                            const uint32_t* ref_start, uint32_t* ref_nbr, int N, hipStream_t s);
+
 int launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+int launch_iota_u32(uint32_t* p, size_t n, hipStream_t s);
 int launch_iota_u64(unsigned long long* p, size_t n, hipStream_t s);
 int launch_scale_last_dim(double* x, int N, int ndim, double f, hipStream_t s);
-// D[i] = fixed[i] ? D0[i] : D0[i] * kappa
+// D[w] = fixed[w] ? D0[w] : D0[w] * kappa
 int launch_diameters(const double* D0, const int8_t* fixed, double kappa, double* D, int N,
+                     hipStream_t s);
+// rk[w] = D_w / 2 with D_w = fixed[w] ? D0[w] : D0[w]*kappa (all particles active)
+int launch_rk_active(const double* D0, const int8_t* fixed, double kappa, double* rk, int N,
                      hipStream_t s);
 // Standalone per-DOF ADAM + position + wrap over supplied forces.
 int launch_adam_pass(const Geometry& g, const double* F, double* m, double* v, const double* x,
                      double* xo, double alpha, double inv_b1, double inv_b2, hipStream_t s);
-// Mean-force numerator of supplied forces into a.result[S_SMAG], [S_MCNT]
-// (a.partials / a.counter / a.result / a.mag_out used); exact = particle order.
+// Mean-force numerator of supplied forces (reference order) into
+// a.result[S_SMAG], [S_MCNT]; exact = particle order.
 int launch_mean_force(const StepArgs& a, const double* F, int exact, hipStream_t s);
-// rk[i] = D_i / 2 with D_i = fixed[i] ? D0[i] : D0[i]*kappa (all particles active)
-int launch_rk_active(const double* D0, const int8_t* fixed, double kappa, double* rk, int N,
-                     hipStream_t s);
-// x_anchor = x
-int launch_copy_f64(const double* src, double* dst, size_t n, hipStream_t s);
 
 }  // namespace rcp

@@ -1,13 +1,18 @@
 // HIP kernels of the MI355X (gfx950) packing engine.
 //
-// Numerics follow the reference engine's 1-thread evaluation order
+// Numerics follow the reference engine's one-thread evaluation order
 // (KD-physics/RCPGenerator python_code/python/src/core/rcp_generator.cpp,
 // cited as RG:<line>): each particle sums its contact forces over partners in
-// ascending index order, which is the order in which the reference's
-// row-ordered half-list loop touches F_i at one OpenMP thread (RG:4086-4139,
-// F_i += for rows i, F_j -= for rows j < i).  Built with -ffp-contract=off so
-// no multiply-add is fused; sqrt and division are the correctly rounded
-// IEEE operations.  FP64 throughout, no MFMA (nothing here is a contraction).
+// ascending reference index (tag) order, which is the order in which the
+// reference's row-ordered half-list loop touches F_i at one OpenMP thread
+// (RG:4086-4139: F_i += for rows i, F_j -= for rows j < i).  Built with
+// -ffp-contract=off so no multiply-add is fused; sqrt and division are the
+// correctly rounded IEEE operations.  FP64 throughout; no MFMA (nothing here
+// is a dense contraction -- the kernels are bandwidth and latency bound).
+//
+// State is stored in a working order that is the cell order of the last
+// re-binning (see rcp_kernels.h), so one wavefront holds spatially adjacent
+// particles: own-data access is coalesced and neighbour reads hit L2.
 #include <hip/hip_runtime.h>
 
 #include "rcp_kernels.h"
@@ -18,9 +23,16 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
+constexpr int kDirectBlock = 128;
+constexpr int kDirectCap = 24;      // contacts buffered per particle in direct mode
 constexpr double kBeta1 = 0.9;
 constexpr double kBeta2 = 0.999;
 constexpr double kEps = 1e-8;
+
+__device__ __forceinline__ bool is_max_slot(int k)
+{
+    return k == S_MAXOV || k == S_MAXDISP2 || k == S_MAXSTEP2;
+}
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -36,7 +48,7 @@ __device__ __forceinline__ double wave_max(double v)
     return v;
 }
 
-// Read lane k's double (k uniform across the wave).
+// Read lane k's value (k uniform across the wave).
 __device__ __forceinline__ double readlane_f64(double v, int k)
 {
     unsigned long long b = __double_as_longlong(v);
@@ -47,6 +59,12 @@ __device__ __forceinline__ double readlane_f64(double v, int k)
 __device__ __forceinline__ int readlane_i32(int v, int k)
 {
     return __builtin_amdgcn_readlane(v, k);
+}
+
+__device__ __forceinline__ double mic(double delta, double inv_box, double box)
+{
+    // delta -= nearbyint(delta * inv_box) * box   (RG:4097-4101)
+    return delta - rint(delta * inv_box) * box;
 }
 
 // One DOF of the fused ADAM + position + wrap pass (RG:6507-6518).
@@ -65,21 +83,24 @@ __device__ __forceinline__ double adam_dof(double Fk, double& m, double& v, doub
     return xv;
 }
 
-__device__ __forceinline__ double mic(double delta, double inv_box, double box)
+__device__ __forceinline__ int cell_coord(double x, double L, double scale, int nc)
 {
-    // delta -= nearbyint(delta * inv_box) * box   (RG:4097-4101)
-    return delta - rint(delta * inv_box) * box;
+    double xw = x - L * floor(x / L);
+    int k = (int)floor(xw * scale);
+    if (k < 0) k = 0;
+    if (k >= nc) k = nc - 1;
+    return k;
 }
 
-// Block-wide deterministic reduction of S_NUM scalars (sums except the two
-// max slots); thread 0 of the block receives the result in out[].
+// Block-wide deterministic reduction of S_NUM scalars; thread 0 receives
+// the block totals.
+template <int BLOCK>
 __device__ void block_reduce(double (&v)[S_NUM], double* out)
 {
-    __shared__ double sh[kBlock / kWave][S_NUM];
+    __shared__ double sh[BLOCK / kWave][S_NUM];
     const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
 #pragma unroll
-    for (int k = 0; k < S_NUM; ++k)
-        v[k] = (k == S_MAXOV || k == S_MAXDISP2) ? wave_max(v[k]) : wave_sum(v[k]);
+    for (int k = 0; k < S_NUM; ++k) v[k] = is_max_slot(k) ? wave_max(v[k]) : wave_sum(v[k]);
     if (lane == 0) {
 #pragma unroll
         for (int k = 0; k < S_NUM; ++k) sh[w][k] = v[k];
@@ -89,8 +110,8 @@ __device__ void block_reduce(double (&v)[S_NUM], double* out)
 #pragma unroll
         for (int k = 0; k < S_NUM; ++k) {
             double acc = sh[0][k];
-            for (int q = 1; q < (int)(blockDim.x / kWave); ++q) {
-                if (k == S_MAXOV || k == S_MAXDISP2) acc = (sh[q][k] > acc) ? sh[q][k] : acc;
+            for (int q = 1; q < BLOCK / kWave; ++q) {
+                if (is_max_slot(k)) acc = (sh[q][k] > acc) ? sh[q][k] : acc;
                 else acc += sh[q][k];
             }
             out[k] = acc;
@@ -99,8 +120,9 @@ __device__ void block_reduce(double (&v)[S_NUM], double* out)
     __syncthreads();
 }
 
-// The last block to finish folds all block partials (block order) into
+// The last block to finish folds all block partials (in block order) into
 // a.result and re-arms the ticket.
+template <int BLOCK>
 __device__ void finish_grid(const StepArgs& a, double (&blk)[S_NUM])
 {
     __shared__ int is_last;
@@ -114,224 +136,403 @@ __device__ void finish_grid(const StepArgs& a, double (&blk)[S_NUM])
     if (!is_last) return;
     __threadfence();
     double v[S_NUM];
+#pragma unroll
     for (int k = 0; k < S_NUM; ++k) v[k] = 0.0;
     for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
         const double* p = a.partials + (size_t)b * S_NUM;
 #pragma unroll
         for (int k = 0; k < S_NUM; ++k) {
             double t = __builtin_nontemporal_load(p + k);
-            if (k == S_MAXOV || k == S_MAXDISP2) v[k] = (t > v[k]) ? t : v[k];
+            if (is_max_slot(k)) v[k] = (t > v[k]) ? t : v[k];
             else v[k] += t;
         }
     }
     double tot[S_NUM];
-    block_reduce(v, tot);
+    block_reduce<BLOCK>(v, tot);
     if (threadIdx.x == 0) {
         for (int k = 0; k < S_NUM; ++k) a.result[k] = tot[k];
         __hip_atomic_store(a.counter, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
-// -------------------------------------------------------------- fused step
-//
-// Per particle i (one thread):
-//   contact forces over the ascending full row (RG:4086-4139 contact law),
-//   flat walls and the curved container (RG:4198-4257), |F_i| for the mean
-//   force (RG:769-803), then the fused per-DOF ADAM + position + wrap
-//   (RG:6499-6520) into x_out, and the squared minimum-image displacement
-//   from the list anchor.
+// ------------------------------------------------------ per-particle parts
+
 template <int ND>
-__global__ __launch_bounds__(kBlock) void k_step(StepArgs a)
+struct Own {
+    double x[ND];
+    double D, rk;
+    uint32_t tag;
+    bool fixed;
+};
+
+template <int ND>
+__device__ __forceinline__ void load_own(const StepArgs& a, int q, Own<ND>& o)
 {
-    const int N = a.g.N;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) o.x[d] = a.x_in[(size_t)q * ND + d];
+    o.fixed = a.fixed && a.fixed[q];
+    const double D0 = a.D0[q];
+    o.D = o.fixed ? D0 : D0 * a.kappa;
+    o.rk = a.rk ? a.rk[q] : (o.fixed ? 0.0 : o.D * 0.5);
+    o.tag = a.tag[q];
+}
+
+// Contact of particle o with partner qj (RG:4086-4139): adds the force to F
+// and, when the partner's tag is larger, the pair terms to acc.  Returns
+// whether the pair terms were counted (exact mode records them).
+template <int ND>
+__device__ __forceinline__ bool contact(const StepArgs& a, const Own<ND>& o, uint32_t qj,
+                                        uint32_t tj, double (&F)[ND], double (&acc)[S_NUM],
+                                        double* rec)
+{
+    const double* xj = a.x_in + (size_t)qj * ND;
+    const bool fj = a.fixed && a.fixed[qj];
+    const double D0j = a.D0[qj];
+    const double Dj = fj ? D0j : D0j * a.kappa;
+    const double r = 0.5 * (o.D + Dj);
+    double dl[ND];
+    double d2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        double delta = xj[d] - o.x[d];
+        delta = mic(delta, a.g.inv_box[d], a.g.box[d]);
+        dl[d] = delta;
+        d2 += delta * delta;
+    }
+    if (!(d2 < r * r)) return false;
+    const double dist = sqrt(d2);
+    const double fm = -(r / dist - 1.0);
+#pragma unroll
+    for (int d = 0; d < ND; ++d) F[d] += fm * dl[d];
+    if (tj <= o.tag) return false;
+    const double ov = 1.0 - dist / r;
+    acc[S_U] += ov;
+    acc[S_CNT] += 1.0;
+    acc[S_LC] += r;
+    acc[S_MAXOV] = (ov > acc[S_MAXOV]) ? ov : acc[S_MAXOV];
+    const double rkj = a.rk ? a.rk[qj] : (fj ? 0.0 : Dj * 0.5);
+    const double dkt = (a.use_rk ? (o.rk + rkj) : r) * (dist - r);
+    acc[S_DK] = acc[S_DK] + dkt;
+    acc[S_FM] = acc[S_FM] + fm;
+    if (rec) {
+        rec[0] = ov;
+        rec[1] = r;
+        rec[2] = dkt;
+        rec[3] = fm;
+    }
+    return true;
+}
+
+// Walls, curved container, mean-force term, ADAM + wrap and displacements
+// for particle q (RG:4198-4257, 769-803, 6499-6520).
+template <int ND>
+__device__ __forceinline__ void particle_tail(const StepArgs& a, int q, const Own<ND>& o,
+                                              double (&F)[ND], double (&acc)[S_NUM])
+{
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        if (a.g.walls[d] == 1) {
+            const double rw = o.D / 2;
+            for (int wall = 0; wall < 2; ++wall) {
+                const double dxw = a.g.box[d] * (double)wall - o.x[d];
+                const double dist = fabs(dxw);
+                if (dist < rw) {
+                    const double fmw = -2.0 * (rw / dist - 1.0);
+                    acc[S_U] += 2.0 * (1.0 - dist / rw);
+                    acc[S_CNT] += 1.0;
+                    F[d] += fmw * dxw;
+                    acc[S_DK] = acc[S_DK] + (2.0 * (a.use_rk ? o.rk : rw)) * (dist - rw);
+                    acc[S_FM] = acc[S_FM] + fmw;
+                }
+            }
+        }
+    }
+    if (a.g.curved_T > 0) {
+        const int T = a.g.curved_T;
+        const double rc = o.D / 2;
+        const double R = a.g.box[0] / 2;
+        double dij = 0;
+#pragma unroll
+        for (int M = 0; M < ND; ++M)
+            if (M < T) dij = dij + (o.x[M] - R) * (o.x[M] - R);
+        dij = sqrt(dij);
+        double delta = R - dij;
+        if (delta < rc) {
+            delta = dij;
+            dij = 0;
+            double dx[ND];
+#pragma unroll
+            for (int M = 0; M < ND; ++M) {
+                dx[M] = 0.0;
+                if (M < T) {
+                    dx[M] = fabs(R / delta - 1) * (o.x[M] - R);
+                    dij = dij + dx[M] * dx[M];
+                }
+            }
+            dij = sqrt(dij);
+            const double fmc = -2.0 * fabs(rc / dij - 1.0);
+            acc[S_U] = acc[S_U] + (1 - dij / rc);
+            acc[S_CNT] += 1.0;
+#pragma unroll
+            for (int M = 0; M < ND; ++M)
+                if (M < T) F[M] = F[M] + fmc * dx[M];
+            acc[S_DK] = acc[S_DK] + (2.0 * (a.use_rk ? o.rk : rc)) * (dij - rc);
+            acc[S_FM] = acc[S_FM] + fmc;
+        }
+    }
+    double f2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) f2 += F[d] * F[d];
+    double mag = -1.0;
+    if (f2 > 1.0E-16) {
+        mag = sqrt(f2);
+        acc[S_SMAG] += mag;
+        acc[S_MCNT] += 1.0;
+    }
+    if (a.mag_out) a.mag_out[q] = mag;
+    acc[S_SUMD] = (a.use_rk && !a.rk && o.fixed) ? 0.0 : o.D;
+    if (a.F_out) {
+#pragma unroll
+        for (int d = 0; d < ND; ++d) a.F_out[(size_t)q * ND + d] = F[d];
+    }
+    if (a.do_adam) {
+        double d2a = 0.0, d2s = 0.0;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            const size_t idx = (size_t)q * ND + d;
+            double mm = a.m[idx], vv = a.v[idx];
+            const double xv = adam_dof(F[d], mm, vv, o.x[d], a.g.box[d], a.alpha, a.inv_b1,
+                                       a.inv_b2);
+            a.m[idx] = mm;
+            a.v[idx] = vv;
+            a.x_out[idx] = xv;
+            double dd = xv - o.x[d];
+            dd = mic(dd, a.g.inv_box[d], a.g.box[d]);
+            d2s += dd * dd;
+            if (a.x_anchor) {
+                double da = xv - a.x_anchor[idx];
+                da = mic(da, a.g.inv_box[d], a.g.box[d]);
+                d2a += da * da;
+            }
+        }
+        acc[S_MAXDISP2] = d2a;
+        acc[S_MAXSTEP2] = d2s;
+    }
+}
+
+// ----------------------------------------------------------- list step
+//
+// Thread q walks its row (partners ascending by tag).  In exact mode the
+// pair terms of each slot are recorded for the sequential reduction.
+template <int ND>
+__global__ __launch_bounds__(kBlock) void k_step_list(StepArgs a, uint32_t* vcount)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
     double acc[S_NUM];
 #pragma unroll
     for (int k = 0; k < S_NUM; ++k) acc[k] = 0.0;
-
-    if (i < N) {
-        double xi[ND], F[ND];
+    if (q < a.g.N) {
+        Own<ND> o;
+        load_own<ND>(a, q, o);
+        double F[ND];
 #pragma unroll
-        for (int d = 0; d < ND; ++d) { xi[d] = a.x_in[(size_t)i * ND + d]; F[d] = 0.0; }
-        const bool fi = a.fixed && a.fixed[i];
-        const double D0i = a.D0[i];
-        const double Di = fi ? D0i : D0i * a.kappa;
-        const double rki = a.rk ? a.rk[i] : (fi ? 0.0 : Di * 0.5);
-        const uint32_t e0 = a.row_start[i], e1 = a.row_start[i + 1];
+        for (int d = 0; d < ND; ++d) F[d] = 0.0;
+        const uint32_t e0 = a.row_start[q], e1 = a.row_start[q + 1];
+        uint32_t nvalid = 0;
         for (uint32_t e = e0; e < e1; ++e) {
-            const uint32_t j = a.nbr[e];
-            const double* xj = a.x_in + (size_t)j * ND;
-            const bool fj = a.fixed && a.fixed[j];
-            const double D0j = a.D0[j];
-            const double Dj = fj ? D0j : D0j * a.kappa;
-            const double r = 0.5 * (Di + Dj);
-            double dl[ND];
-            double d2 = 0.0;
-#pragma unroll
-            for (int d = 0; d < ND; ++d) {
-                double delta = xj[d] - xi[d];
-                delta = mic(delta, a.g.inv_box[d], a.g.box[d]);
-                dl[d] = delta;
-                d2 += delta * delta;
-            }
-            bool valid = false;
-            double ov = 0.0, dkt = 0.0, fm = 0.0;
-            if (d2 < r * r) {
-                const double dist = sqrt(d2);
-                fm = -(r / dist - 1.0);
-#pragma unroll
-                for (int d = 0; d < ND; ++d) F[d] += fm * dl[d];
-                if (j > (uint32_t)i) {
-                    valid = true;
-                    ov = 1.0 - dist / r;
-                    acc[S_U] += ov;
-                    acc[S_CNT] += 1.0;
-                    acc[S_LC] += r;
-                    acc[S_MAXOV] = (ov > acc[S_MAXOV]) ? ov : acc[S_MAXOV];
-                    const double rkj = a.rk ? a.rk[j] : (fj ? 0.0 : Dj * 0.5);
-                    dkt = (a.use_rk ? (rki + rkj) : r) * (dist - r);
-                    acc[S_DK] = acc[S_DK] + dkt;
-                    acc[S_FM] = acc[S_FM] + fm;
-                }
-            }
+            const uint32_t qj = a.nbr[e];
+            const uint32_t tj = a.tag[qj];
             if (a.contrib) {
-                double* c = a.contrib + (size_t)e * 4;
-                c[0] = ov;
-                c[1] = valid ? r : 0.0;
-                c[2] = dkt;
-                c[3] = fm;
+                double* rec = a.contrib + (size_t)e * 4;
+                if (contact<ND>(a, o, qj, tj, F, acc, rec)) ++nvalid;
+                else rec[1] = 0.0;
+            } else {
+                contact<ND>(a, o, qj, tj, F, acc, nullptr);
             }
         }
-        // Flat walls (RG:4200-4220); walls[d] == 1 only.
-#pragma unroll
-        for (int d = 0; d < ND; ++d) {
-            if (a.g.walls[d] == 1) {
-                const double rw = Di / 2;
-                for (int wall = 0; wall < 2; ++wall) {
-                    const double dxw = a.g.box[d] * (double)wall - xi[d];
-                    const double dist = fabs(dxw);
-                    if (dist < rw) {
-                        const double fmw = -2.0 * (rw / dist - 1.0);
-                        acc[S_U] += 2.0 * (1.0 - dist / rw);
-                        acc[S_CNT] += 1.0;
-                        F[d] += fmw * dxw;
-                        acc[S_DK] = acc[S_DK] + (2.0 * (a.use_rk ? rki : rw)) * (dist - rw);
-                        acc[S_FM] = acc[S_FM] + fmw;
-                    }
-                }
-            }
-        }
-        // Curved container over the first T axes (RG:4222-4257).
-        if (a.g.curved_T > 0) {
-            const int T = a.g.curved_T;
-            const double rc = Di / 2;
-            const double R = a.g.box[0] / 2;
-            double dij = 0;
-#pragma unroll
-            for (int M = 0; M < ND; ++M)
-                if (M < T) dij = dij + (xi[M] - R) * (xi[M] - R);
-            dij = sqrt(dij);
-            double delta = R - dij;
-            if (delta < rc) {
-                delta = dij;
-                dij = 0;
-                double dx[ND];
-#pragma unroll
-                for (int M = 0; M < ND; ++M) {
-                    dx[M] = 0.0;
-                    if (M < T) {
-                        dx[M] = fabs(R / delta - 1) * (xi[M] - R);
-                        dij = dij + dx[M] * dx[M];
-                    }
-                }
-                dij = sqrt(dij);
-                const double fmc = -2.0 * fabs(rc / dij - 1.0);
-                acc[S_U] = acc[S_U] + (1 - dij / rc);
-                acc[S_CNT] += 1.0;
-#pragma unroll
-                for (int M = 0; M < ND; ++M)
-                    if (M < T) F[M] = F[M] + fmc * dx[M];
-                acc[S_DK] = acc[S_DK] + (2.0 * (a.use_rk ? rki : rc)) * (dij - rc);
-                acc[S_FM] = acc[S_FM] + fmc;
-            }
-        }
-        // Mean-force contribution (RG:784-796).
-        double f2 = 0.0;
-#pragma unroll
-        for (int d = 0; d < ND; ++d) f2 += F[d] * F[d];
-        double mag = -1.0;
-        if (f2 > 1.0E-16) {
-            mag = sqrt(f2);
-            acc[S_SMAG] += mag;
-            acc[S_MCNT] += 1.0;
-        }
-        if (a.mag_out) a.mag_out[i] = mag;
-        acc[S_SUMD] = (a.use_rk && !a.rk && fi) ? 0.0 : Di;
-        if (a.F_out) {
-#pragma unroll
-            for (int d = 0; d < ND; ++d) a.F_out[(size_t)i * ND + d] = F[d];
-        }
-        if (a.do_adam) {
-            double d2a = 0.0;
-#pragma unroll
-            for (int d = 0; d < ND; ++d) {
-                const size_t idx = (size_t)i * ND + d;
-                double mm = a.m[idx], vv = a.v[idx];
-                const double xv = adam_dof(F[d], mm, vv, xi[d], a.g.box[d], a.alpha, a.inv_b1,
-                                           a.inv_b2);
-                a.m[idx] = mm;
-                a.v[idx] = vv;
-                a.x_out[idx] = xv;
-                double dd = xv - a.x_anchor[idx];
-                dd = mic(dd, a.g.inv_box[d], a.g.box[d]);
-                d2a += dd * dd;
-            }
-            acc[S_MAXDISP2] = d2a;
-        }
+        if (vcount) vcount[o.tag] = nvalid;
+        particle_tail<ND>(a, q, o, F, acc);
     }
     double blk[S_NUM];
-    block_reduce(acc, blk);
-    finish_grid(a, blk);
+    block_reduce<kBlock>(acc, blk);
+    finish_grid<kBlock>(a, blk);
+}
+
+// ---------------------------------------------------------- direct step
+//
+// No neighbour list: thread q scans the 3^d stencil of its cell in the
+// current working order (cells are contiguous ranges), buffers its contacts
+// as (tag << 32 | position) in LDS, sorts them by tag, then accumulates in
+// that order.  A particle with more than kDirectCap contacts takes the
+// selection path: repeated stencil scans for the next larger tag.
+template <int ND, class Fn>
+__device__ __forceinline__ void for_each_candidate(const StepArgs& a, const double* x, Fn&& fn)
+{
+    int ci[ND], span[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        ci[d] = cell_coord(x[d], a.g.box[d], a.grid.cell_scale[d], a.grid.nc[d]);
+        span[d] = a.grid.nc[d] >= 3 ? 3 : 1;
+    }
+    int nst = 1;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) nst *= span[d];
+    for (int s = 0; s < nst; ++s) {
+        int t = s;
+        long long cc = 0, stride = 1;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            int k = ci[d];
+            if (span[d] == 3) {
+                k += t % 3 - 1;
+                t /= 3;
+                if (k < 0) k += a.grid.nc[d];
+                if (k >= a.grid.nc[d]) k -= a.grid.nc[d];
+            }
+            cc += (long long)k * stride;
+            stride *= a.grid.nc[d];
+        }
+        const uint32_t q0 = a.cell_start[cc], q1 = a.cell_start[cc + 1];
+        for (uint32_t qj = q0; qj < q1; ++qj) fn(qj);
+    }
+}
+
+template <int ND>
+__device__ __forceinline__ bool overlaps(const StepArgs& a, const Own<ND>& o, uint32_t qj)
+{
+    const double* xj = a.x_in + (size_t)qj * ND;
+    const bool fj = a.fixed && a.fixed[qj];
+    const double D0j = a.D0[qj];
+    const double Dj = fj ? D0j : D0j * a.kappa;
+    const double r = 0.5 * (o.D + Dj);
+    double d2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        double delta = xj[d] - o.x[d];
+        delta = mic(delta, a.g.inv_box[d], a.g.box[d]);
+        d2 += delta * delta;
+    }
+    return d2 < r * r;
+}
+
+template <int ND>
+__global__ __launch_bounds__(kDirectBlock) void k_step_direct(StepArgs a)
+{
+    __shared__ unsigned long long buf[kDirectCap][kDirectBlock];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tid = threadIdx.x;
+    double acc[S_NUM];
+#pragma unroll
+    for (int k = 0; k < S_NUM; ++k) acc[k] = 0.0;
+    if (q < a.g.N) {
+        Own<ND> o;
+        load_own<ND>(a, q, o);
+        double F[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) F[d] = 0.0;
+        int n = 0;
+        for_each_candidate<ND>(a, o.x, [&](uint32_t qj) {
+            if (qj == (uint32_t)q) return;
+            if (!overlaps<ND>(a, o, qj)) return;
+            if (n < kDirectCap)
+                buf[n][tid] = ((unsigned long long)a.tag[qj] << 32) | (unsigned long long)qj;
+            ++n;
+        });
+        if (n <= kDirectCap) {
+            for (int p = 1; p < n; ++p) {
+                const unsigned long long key = buf[p][tid];
+                int k = p;
+                while (k > 0 && buf[k - 1][tid] > key) {
+                    buf[k][tid] = buf[k - 1][tid];
+                    --k;
+                }
+                buf[k][tid] = key;
+            }
+            for (int p = 0; p < n; ++p) {
+                const unsigned long long key = buf[p][tid];
+                contact<ND>(a, o, (uint32_t)(key & 0xffffffffull), (uint32_t)(key >> 32), F, acc,
+                            nullptr);
+            }
+        } else {
+            acc[S_OVERFLOW] += 1.0;
+            long long last = -1;
+            while (true) {
+                unsigned long long best = ~0ull;
+                for_each_candidate<ND>(a, o.x, [&](uint32_t qj) {
+                    if (qj == (uint32_t)q) return;
+                    const uint32_t tj = a.tag[qj];
+                    if ((long long)tj <= last) return;
+                    const unsigned long long key =
+                        ((unsigned long long)tj << 32) | (unsigned long long)qj;
+                    if (key >= best) return;
+                    if (overlaps<ND>(a, o, qj)) best = key;
+                });
+                if (best == ~0ull) break;
+                contact<ND>(a, o, (uint32_t)(best & 0xffffffffull), (uint32_t)(best >> 32), F, acc,
+                            nullptr);
+                last = (long long)(best >> 32);
+            }
+        }
+        particle_tail<ND>(a, q, o, F, acc);
+    }
+    double blk[S_NUM];
+    block_reduce<kDirectBlock>(acc, blk);
+    finish_grid<kDirectBlock>(a, blk);
 }
 
 // -------------------------------------------------------- exact reduction
 //
-// One wavefront re-sums the scalar terms in the reference's 1-thread order:
-// pair terms row by row (rows ascending, partners ascending, j > i only;
-// RG:4106-4138), then walls particle by particle (RG:4198-4257), then
-// sum(D) (RG:4262-4265) and the mean-force numerator (RG:784-796).  Terms
-// are computed in parallel (k_step stored the pair terms per row slot) and
-// folded by lane 0..63 readlane in order, so every addition rounds exactly
-// as the reference's sequential loop does.
+// The recorded pair terms are first compacted into reference order (rows by
+// ascending tag of the owner, partners by ascending tag), then one
+// wavefront folds every term in the reference's one-thread order: pairs
+// (RG:4106-4138), walls particle by particle (RG:4198-4257), sum(D)
+// (RG:4262-4265) and the mean-force numerator (RG:784-796).  Lanes load 64
+// terms at once; lane-by-lane readlane keeps the additions sequential.
+__global__ __launch_bounds__(kBlock) void k_exact_compact(StepArgs a, const uint32_t* ref_start,
+                                                          double* ref_terms)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.g.N) return;
+    uint32_t dst = ref_start[a.tag[q]];
+    for (uint32_t e = a.row_start[q]; e < a.row_start[q + 1]; ++e) {
+        const double* rec = a.contrib + (size_t)e * 4;
+        if (rec[1] != 0.0) {
+            double* t = ref_terms + (size_t)dst * 4;
+            t[0] = rec[0];
+            t[1] = rec[1];
+            t[2] = rec[2];
+            t[3] = rec[3];
+            ++dst;
+        }
+    }
+}
+
 template <int ND>
-__global__ __launch_bounds__(kWave) void k_exact_reduce(StepArgs a)
+__global__ __launch_bounds__(kWave) void k_exact_reduce(StepArgs a, const uint32_t* ref_start,
+                                                        const double* ref_terms)
 {
     const int lane = threadIdx.x;
     const int N = a.g.N;
     double U = 0.0, Lc = 0.0, dk = 0.0, Fm = 0.0, maxov = 0.0, cnt = 0.0;
-    const uint32_t total = a.row_start[N];
+    const uint32_t total = ref_start[N];
     for (uint32_t base = 0; base < total; base += kWave) {
         const uint32_t e = base + lane;
         double ov = 0.0, r = 0.0, dkt = 0.0, fm = 0.0;
         if (e < total) {
-            const double* c = a.contrib + (size_t)e * 4;
-            ov = c[0]; r = c[1]; dkt = c[2]; fm = c[3];
+            const double* t = ref_terms + (size_t)e * 4;
+            ov = t[0]; r = t[1]; dkt = t[2]; fm = t[3];
         }
         const int lim = (total - base < (uint32_t)kWave) ? (int)(total - base) : kWave;
         for (int k = 0; k < lim; ++k) {
-            const double rk = readlane_f64(r, k);
-            if (rk != 0.0) {
-                const double ok = readlane_f64(ov, k);
-                U += ok;
-                cnt += 1.0;
-                Lc += rk;
-                if (ok > maxov) maxov = ok;
-                dk = dk + readlane_f64(dkt, k);
-                Fm = Fm + readlane_f64(fm, k);
-            }
+            const double ok = readlane_f64(ov, k);
+            U += ok;
+            cnt += 1.0;
+            Lc += readlane_f64(r, k);
+            if (ok > maxov) maxov = ok;
+            dk = dk + readlane_f64(dkt, k);
+            Fm = Fm + readlane_f64(fm, k);
         }
     }
-    // walls and curved container, in particle order
     constexpr int NT = 2 * ND + 1;
     for (int base = 0; base < N; base += kWave) {
         const int i = base + lane;
@@ -340,12 +541,13 @@ __global__ __launch_bounds__(kWave) void k_exact_reduce(StepArgs a)
 #pragma unroll
         for (int t = 0; t < NT; ++t) { tu[t] = td[t] = tf[t] = 0.0; tv[t] = 0; }
         if (i < N) {
+            const uint32_t w = a.inv_tag[i];
             double xi[ND];
 #pragma unroll
-            for (int d = 0; d < ND; ++d) xi[d] = a.x_in[(size_t)i * ND + d];
-            const bool fi = a.fixed && a.fixed[i];
-            const double Di = fi ? a.D0[i] : a.D0[i] * a.kappa;
-            const double rki = a.rk ? a.rk[i] : (fi ? 0.0 : Di * 0.5);
+            for (int d = 0; d < ND; ++d) xi[d] = a.x_in[(size_t)w * ND + d];
+            const bool fi = a.fixed && a.fixed[w];
+            const double Di = fi ? a.D0[w] : a.D0[w] * a.kappa;
+            const double rki = a.rk ? a.rk[w] : (fi ? 0.0 : Di * 0.5);
 #pragma unroll
             for (int d = 0; d < ND; ++d) {
                 if (a.g.walls[d] == 1) {
@@ -405,17 +607,17 @@ __global__ __launch_bounds__(kWave) void k_exact_reduce(StepArgs a)
             }
         }
     }
-    // sum(D) or the active-only sum, then the mean-force numerator
     double sumD = 0.0, smag = 0.0, mcnt = 0.0;
     for (int base = 0; base < N; base += kWave) {
         const int i = base + lane;
         double Di = 0.0, mag = -1.0;
         int act = 0;
         if (i < N) {
-            const bool fi = a.fixed && a.fixed[i];
-            Di = fi ? a.D0[i] : a.D0[i] * a.kappa;
+            const uint32_t w = a.inv_tag[i];
+            const bool fi = a.fixed && a.fixed[w];
+            Di = fi ? a.D0[w] : a.D0[w] * a.kappa;
             act = !(a.use_rk && !a.rk && fi);
-            mag = a.mag_out[i];
+            mag = a.mag_out[w];
         }
         const int lim = (N - base < kWave) ? (N - base) : kWave;
         for (int k = 0; k < lim; ++k) {
@@ -437,112 +639,159 @@ __global__ __launch_bounds__(kWave) void k_exact_reduce(StepArgs a)
     }
 }
 
-// ------------------------------------------------------------ cell lists
+// ---------------------------------------------------------------- binning
 
 template <int ND>
-__global__ __launch_bounds__(kBlock) void k_cell_assign(ListArgs a)
+__global__ __launch_bounds__(kBlock) void k_bin(BinArgs a)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.g.N) return;
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= a.g.N) return;
     long long c = 0, stride = 1;
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
-        const double L = a.g.box[d];
-        double xw = a.x[(size_t)i * ND + d];
-        xw = xw - L * floor(xw / L);
-        int k = (int)floor(xw * a.grid.cell_scale[d]);
-        if (k < 0) k = 0;
-        if (k >= a.grid.nc[d]) k = a.grid.nc[d] - 1;
+        const int k = cell_coord(a.x[(size_t)w * ND + d], a.g.box[d], a.grid.cell_scale[d],
+                                 a.grid.nc[d]);
         c += (long long)k * stride;
         stride *= a.grid.nc[d];
     }
-    a.cell_of[i] = (uint32_t)c;
-    a.slot[i] = atomicAdd(&a.cell_count[c], 1u);
+    a.cell_of[w] = (uint32_t)c;
+    a.slot[w] = atomicAdd(&a.cell_count[c], 1u);
 }
 
-__global__ __launch_bounds__(kBlock) void k_cell_scatter(ListArgs a)
+__global__ __launch_bounds__(kBlock) void k_place(BinArgs a)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.g.N) return;
-    a.items[a.cell_start[a.cell_of[i]] + a.slot[i]] = (uint32_t)i;
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= a.g.N) return;
+    a.perm[a.cell_start[a.cell_of[w]] + a.slot[w]] = (uint32_t)w;
 }
 
-// Candidate rows.  Predicate: include j iff !(d2 >= rc*rc) with
-// rc = (Di + Dj) * 0.5 + t_shell (the kd-tree test, RG:3276-3297), MIC on
-// every axis.  FILL=false counts, FILL=true writes and sorts the row.
+// One thread per cell: order the cell's members by tag (deterministic).
+__global__ __launch_bounds__(kBlock) void k_cell_order(BinArgs a)
+{
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.grid.total) return;
+    const uint32_t q0 = a.cell_start[c], q1 = a.cell_start[c + 1];
+    for (uint32_t p = q0 + 1; p < q1; ++p) {
+        const uint32_t w = a.perm[p];
+        const uint32_t t = a.tag[w];
+        uint32_t k = p;
+        while (k > q0 && a.tag[a.perm[k - 1]] > t) {
+            a.perm[k] = a.perm[k - 1];
+            --k;
+        }
+        a.perm[k] = w;
+    }
+}
+
+template <int ND>
+__global__ __launch_bounds__(kBlock) void k_permute(BinArgs a)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.g.N) return;
+    const uint32_t w = a.perm[q];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        a.x_o[(size_t)q * ND + d] = a.x[(size_t)w * ND + d];
+        a.m_o[(size_t)q * ND + d] = a.m[(size_t)w * ND + d];
+        a.v_o[(size_t)q * ND + d] = a.v[(size_t)w * ND + d];
+    }
+    a.D0_o[q] = a.D0[w];
+    const uint32_t t = a.tag[w];
+    a.tag_o[q] = t;
+    a.inv_tag[t] = (uint32_t)q;
+    a.origin_o[q] = a.origin[w];
+    if (a.fixed) a.fixed_o[q] = a.fixed[w];
+}
+
+// ------------------------------------------------------------------ lists
+
+// Candidate rows over the cell stencil.  Predicate: include j iff
+// !(d2 >= rc*rc), rc = (Di + Dj)*0.5 + t_shell (the kd-tree test,
+// RG:3276-3297) or, for the sort+walk form (RG:1357),
+// rc = (Di*0.5 + t_shell) + (double)(float)(Dj*0.5) with i the lower tag.
+// FILL=false counts, FILL=true writes the row sorted by partner tag.
 template <int ND, bool FILL>
 __global__ __launch_bounds__(kBlock) void k_rows(ListArgs a)
 {
-    const int N = a.g.N;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.g.N) return;
     double xi[ND];
 #pragma unroll
-    for (int d = 0; d < ND; ++d) xi[d] = a.x[(size_t)i * ND + d];
-    const bool fi = a.fixed && a.fixed[i];
-    const double Di = fi ? a.D0[i] : a.D0[i] * a.kappa;
+    for (int d = 0; d < ND; ++d) xi[d] = a.x[(size_t)q * ND + d];
+    const bool fi = a.fixed && a.fixed[q];
+    const double Di = fi ? a.D0[q] : a.D0[q] * a.kappa;
+    const uint32_t ti = a.tag[q];
     int ci[ND], span[ND];
-    {
-        uint32_t c = a.cell_of[i];
 #pragma unroll
-        for (int d = 0; d < ND; ++d) {
-            ci[d] = (int)(c % (uint32_t)a.grid.nc[d]);
-            c /= (uint32_t)a.grid.nc[d];
-            span[d] = a.grid.nc[d] >= 3 ? 3 : 1;
-        }
+    for (int d = 0; d < ND; ++d) {
+        ci[d] = cell_coord(xi[d], a.g.box[d], a.grid.cell_scale[d], a.grid.nc[d]);
+        span[d] = a.grid.nc[d] >= 3 ? 3 : 1;
     }
     int nst = 1;
 #pragma unroll
     for (int d = 0; d < ND; ++d) nst *= span[d];
     uint32_t count = 0;
-    const uint32_t base = FILL ? a.row_start[i] : 0;
+    const uint32_t base = FILL ? a.row_start[q] : 0;
     for (int s = 0; s < nst; ++s) {
         int t = s;
         long long cc = 0, stride = 1;
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
-            int o = 0;
-            if (span[d] == 3) { o = t % 3 - 1; t /= 3; }
-            int k = ci[d] + o;
-            if (k < 0) k += a.grid.nc[d];
-            if (k >= a.grid.nc[d]) k -= a.grid.nc[d];
+            int k = ci[d];
+            if (span[d] == 3) {
+                k += t % 3 - 1;
+                t /= 3;
+                if (k < 0) k += a.grid.nc[d];
+                if (k >= a.grid.nc[d]) k -= a.grid.nc[d];
+            }
             cc += (long long)k * stride;
             stride *= a.grid.nc[d];
         }
         const uint32_t q0 = a.cell_start[cc], q1 = a.cell_start[cc + 1];
-        for (uint32_t q = q0; q < q1; ++q) {
-            const uint32_t j = a.items[q];
-            if (j == (uint32_t)i) continue;
-            if (a.half && j < (uint32_t)i) continue;
-            const bool fj = a.fixed && a.fixed[j];
-            const double Dj = fj ? a.D0[j] : a.D0[j] * a.kappa;
-            const double rc = (Di + Dj) * 0.5 + a.t_shell;
+        for (uint32_t qj = q0; qj < q1; ++qj) {
+            if (qj == (uint32_t)q) continue;
+            const uint32_t tj = a.tag[qj];
+            if (a.half && tj < ti) continue;
+            const bool fj = a.fixed && a.fixed[qj];
+            const double Dj = fj ? a.D0[qj] : a.D0[qj] * a.kappa;
+            double rc;
+            if (a.sortwalk_rc) {
+                const double Dlo = ti < tj ? Di : Dj, Dhi = ti < tj ? Dj : Di;
+                rc = (Dlo * 0.5 + a.t_shell) + (double)(float)(Dhi * 0.5);
+            } else {
+                rc = (Di + Dj) * 0.5 + a.t_shell;
+            }
             double d2 = 0.0;
 #pragma unroll
             for (int d = 0; d < ND; ++d) {
-                double dd = a.x[(size_t)j * ND + d] - xi[d];
+                double dd = a.x[(size_t)qj * ND + d] - xi[d];
                 dd = mic(dd, a.g.inv_box[d], a.g.box[d]);
                 d2 += dd * dd;
             }
             if (d2 >= rc * rc) continue;
-            if (FILL) a.nbr[base + count] = j;
+            if (FILL) a.nbr[base + count] = qj;
             ++count;
         }
     }
     if (!FILL) {
-        a.row_count[i] = count;
+        a.row_count[q] = count;
     } else {
+        // rows are short: insertion sort by partner tag in place
         uint32_t* row = a.nbr + base;
         for (uint32_t p = 1; p < count; ++p) {
-            const uint32_t key = row[p];
-            uint32_t q = p;
-            while (q > 0 && row[q - 1] > key) { row[q] = row[q - 1]; --q; }
-            row[q] = key;
+            const uint32_t qk = row[p];
+            const uint32_t tk = a.tag[qk];
+            uint32_t k = p;
+            while (k > 0 && a.tag[row[k - 1]] > tk) {
+                row[k] = row[k - 1];
+                --k;
+            }
+            row[k] = qk;
         }
     }
 }
 
-// ------------------------------------------------------------------ scan
+// ------------------------------------------------------------------- scan
 
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kBlock * kScanItems;
@@ -559,7 +808,6 @@ __global__ __launch_bounds__(kBlock) void k_scan_tile(const uint32_t* in, uint32
         v[k] = (b0 + k < n) ? in[b0 + k] : 0u;
         tsum += v[k];
     }
-    // inclusive wave scan of thread sums
     const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
     uint32_t incl = tsum;
     for (int o = 1; o < kWave; o <<= 1) {
@@ -598,7 +846,7 @@ __global__ void k_set_total(uint32_t* out, size_t n, const uint32_t* src)
     out[n] = src[0];
 }
 
-// ----------------------------------------------------------- radix sort
+// ------------------------------------------------------------- radix sort
 
 __device__ __forceinline__ unsigned long long ordered_bits(double x)
 {
@@ -620,33 +868,55 @@ __device__ __forceinline__ unsigned long long morton_split21(unsigned long long 
     return v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_reorder_keys(const double* x, int N, int ndim,
-                                                          double b0, double b1, double b2,
-                                                          unsigned long long* keys,
-                                                          uint32_t* vals)
+// Reorder key of a particle (RG:1127-1150 Morton-21 with clamp to
+// [0, box - 1e-12]; RG:1084-1105 x column otherwise).
+__device__ __forceinline__ unsigned long long reorder_key(const double* xp, int ndim,
+                                                          const Geometry& g)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    unsigned long long key;
     if (ndim == 3) {
-        // RG:1127-1150: clamp to [0, box - 1e-12], quantise to 21 bits
-        const double box[3] = {b0, b1, b2};
         unsigned long long code = 0;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            const double scale = ((double)(1u << 21) - 1.0) / box[d];
-            const double hi = box[d] - 1e-12, v = x[(size_t)i * 3 + d];
+            const double scale = ((double)(1u << 21) - 1.0) / g.box[d];
+            const double hi = g.box[d] - 1e-12, v = xp[d];
             const double m = (v < hi) ? v : hi;
             const double c = (0.0 < m) ? m : 0.0;
-            const unsigned q = (unsigned)(c * scale);
-            code |= morton_split21(q) << d;
+            const unsigned qv = (unsigned)(c * scale);
+            code |= morton_split21(qv) << d;
         }
-        key = code;
-    } else {
-        key = ordered_bits(x[(size_t)i * ndim]);
+        return code;
     }
-    keys[i] = key;
-    vals[i] = (uint32_t)i;
+    return ordered_bits(xp[0]);
+}
+
+// keys in the current reference order (slot k holds the particle with tag k)
+__global__ __launch_bounds__(kBlock) void k_ref_keys(Geometry g, const double* x,
+                                                      const uint32_t* inv_tag,
+                                                      unsigned long long* keys, uint32_t* vals)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.N) return;
+    const uint32_t w = inv_tag ? inv_tag[k] : (uint32_t)k;
+    keys[k] = reorder_key(x + (size_t)w * g.ndim, g.ndim, g);
+    vals[k] = (uint32_t)k;
+}
+
+// rank[old tag] = new tag
+__global__ __launch_bounds__(kBlock) void k_tag_rank(const uint32_t* sorted_old, uint32_t* rank,
+                                                      int N)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < N) rank[sorted_old[r]] = (uint32_t)r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tag_apply(uint32_t* tag, uint32_t* inv_tag,
+                                                       const uint32_t* rank, int N)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= N) return;
+    const uint32_t t = rank[tag[w]];
+    tag[w] = t;
+    inv_tag[t] = (uint32_t)w;
 }
 
 constexpr int kSortItems = 4;
@@ -668,7 +938,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const unsigned long long*
     hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// Stable scatter: elements of the tile are ranked in tile order (striped
+// Stable scatter: elements of a tile are ranked in tile order (striped
 // rounds of 256, wave ballots for the in-wave rank).
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     const unsigned long long* keys, const uint32_t* vals, unsigned long long* keys_out,
@@ -682,7 +952,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (int k = 0; k < kSortItems; ++k) {
 #pragma unroll
-        for (int q = 0; q < kBlock / kWave; ++q) wcnt[q][threadIdx.x] = 0;
+        for (int qq = 0; qq < kBlock / kWave; ++qq) wcnt[qq][threadIdx.x] = 0;
         __syncthreads();
         const size_t e = t0 + (size_t)k * kBlock + threadIdx.x;
         const bool ok = e < n;
@@ -700,7 +970,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
         __syncthreads();
         if (ok) {
             uint32_t off = run[dig] + rank;
-            for (int q = 0; q < w; ++q) off += wcnt[q][dig];
+            for (int qq = 0; qq < w; ++qq) off += wcnt[qq][dig];
             keys_out[off] = key;
             vals_out[off] = vals[e];
         }
@@ -708,14 +978,14 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(
         {
             uint32_t add = 0;
 #pragma unroll
-            for (int q = 0; q < kBlock / kWave; ++q) add += wcnt[q][threadIdx.x];
+            for (int qq = 0; qq < kBlock / kWave; ++qq) add += wcnt[qq][threadIdx.x];
             run[threadIdx.x] += add;
         }
         __syncthreads();
     }
 }
 
-// ---------------------------------------------------- standalone passes
+// ------------------------------------------------------ standalone passes
 
 __global__ __launch_bounds__(kBlock) void k_adam_pass(Geometry g, const double* F, double* m,
                                                       double* v, const double* x, double* xo,
@@ -730,7 +1000,6 @@ __global__ __launch_bounds__(kBlock) void k_adam_pass(Geometry g, const double* 
     v[t] = vv;
 }
 
-// Mean-force numerator (RG:784-796): block sums folded by the last block.
 __global__ __launch_bounds__(kBlock) void k_mean_force(StepArgs a, const double* F)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -752,11 +1021,10 @@ __global__ __launch_bounds__(kBlock) void k_mean_force(StepArgs a, const double*
         if (a.mag_out) a.mag_out[i] = mag;
     }
     double blk[S_NUM];
-    block_reduce(acc, blk);
-    finish_grid(a, blk);
+    block_reduce<kBlock>(acc, blk);
+    finish_grid<kBlock>(a, blk);
 }
 
-// The same numerator folded sequentially in particle order (one wave).
 __global__ __launch_bounds__(kWave) void k_mean_force_exact(StepArgs a)
 {
     const int lane = threadIdx.x;
@@ -776,30 +1044,63 @@ __global__ __launch_bounds__(kWave) void k_mean_force_exact(StepArgs a)
     }
 }
 
-// ----------------------------------------------------------------- small
+// --------------------------------------------------------------- scatters
 
-__global__ void k_gather_f64(const double* src, double* dst, const uint32_t* perm, int N, int w)
+__global__ void k_to_ref_f64(const double* src, double* dst, const uint32_t* tag, int N, int w)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)N * w) return;
     const size_t k = t / w, d = t % w;
-    dst[t] = src[(size_t)perm[k] * w + d];
+    dst[(size_t)tag[k] * w + d] = src[t];
 }
-__global__ void k_gather_u64(const unsigned long long* src, unsigned long long* dst,
-                             const uint32_t* perm, int N)
+__global__ void k_to_ref_u64(const unsigned long long* src, unsigned long long* dst,
+                             const uint32_t* tag, int N)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < N) dst[k] = src[perm[k]];
+    if (k < N) dst[tag[k]] = src[k];
 }
-__global__ void k_gather_i8(const int8_t* src, int8_t* dst, const uint32_t* perm, int N)
+__global__ void k_to_ref_i8(const int8_t* src, int8_t* dst, const uint32_t* tag, int N)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < N) dst[k] = src[perm[k]];
+    if (k < N) dst[tag[k]] = src[k];
 }
+__global__ void k_from_ref_f64(const double* src, double* dst, const uint32_t* tag, int N, int w)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)N * w) return;
+    const size_t k = t / w, d = t % w;
+    dst[t] = src[(size_t)tag[k] * w + d];
+}
+__global__ void k_from_ref_i8(const int8_t* src, int8_t* dst, const uint32_t* tag, int N)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < N) dst[k] = src[tag[k]];
+}
+__global__ void k_rows_ref_count(const uint32_t* row_start, const uint32_t* tag,
+                                 uint32_t* ref_count, int N)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < N) ref_count[tag[q]] = row_start[q + 1] - row_start[q];
+}
+__global__ void k_rows_ref_fill(const uint32_t* row_start, const uint32_t* nbr,
+                                const uint32_t* tag, const uint32_t* ref_start, uint32_t* ref_nbr,
+                                int N)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= N) return;
+    uint32_t dst = ref_start[tag[q]];
+    for (uint32_t e = row_start[q]; e < row_start[q + 1]; ++e) ref_nbr[dst++] = tag[nbr[e]];
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
 {
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) p[k] = v;
+}
+__global__ void k_iota_u32(uint32_t* p, size_t n)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = (uint32_t)k;
 }
 __global__ void k_iota_u64(unsigned long long* p, size_t n)
 {
@@ -821,11 +1122,6 @@ __global__ void k_rk_active(const double* D0, const int8_t* fixed, double kappa,
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < N) rk[k] = ((fixed && fixed[k]) ? D0[k] : D0[k] * kappa) * 0.5;
-}
-__global__ void k_copy_f64(const double* src, double* dst, size_t n)
-{
-    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) dst[k] = src[k];
 }
 
 inline unsigned grid_for(size_t n, int block = kBlock)
@@ -855,29 +1151,49 @@ inline int check_launch()
         default: return 2;                                                             \
     }
 
-int launch_step(const StepArgs& a, hipStream_t s)
+size_t step_partial_blocks(size_t n)
+{
+    return (n + kDirectBlock - 1) / kDirectBlock;
+}
+
+int launch_step_list(const StepArgs& a, hipStream_t s)
 {
     if (a.g.N <= 0) return 0;
     const unsigned g = grid_for(a.g.N);
-    RCP_DISPATCH_ND(a.g.ndim, k_step, <<<g, kBlock, 0, s>>>(a));
+    RCP_DISPATCH_ND(a.g.ndim, k_step_list, <<<g, kBlock, 0, s>>>(a, nullptr));
     return check_launch();
 }
 
-int launch_exact_reduce(const StepArgs& a, hipStream_t s)
+int launch_step_direct(const StepArgs& a, hipStream_t s)
 {
     if (a.g.N <= 0) return 0;
-    RCP_DISPATCH_ND(a.g.ndim, k_exact_reduce, <<<1, kWave, 0, s>>>(a));
+    const unsigned g = grid_for(a.g.N, kDirectBlock);
+    RCP_DISPATCH_ND(a.g.ndim, k_step_direct, <<<g, kDirectBlock, 0, s>>>(a));
     return check_launch();
 }
 
-int launch_cells(ListArgs& a, void* scan_tmp, size_t, hipStream_t s)
+int launch_exact_step(const StepArgs& a, uint32_t* ref_count, uint32_t* ref_start,
+                      double* ref_terms, void* scan_tmp, hipStream_t s)
 {
+    if (a.g.N <= 0) return 0;
+    const unsigned g = grid_for(a.g.N);
+    RCP_DISPATCH_ND(a.g.ndim, k_step_list, <<<g, kBlock, 0, s>>>(a, ref_count));
+    if (launch_scan_u32(ref_count, ref_start, (size_t)a.g.N, scan_tmp, s)) return 1;
+    k_exact_compact<<<g, kBlock, 0, s>>>(a, ref_start, ref_terms);
+    RCP_DISPATCH_ND(a.g.ndim, k_exact_reduce, <<<1, kWave, 0, s>>>(a, ref_start, ref_terms));
+    return check_launch();
+}
+
+int launch_rebin(BinArgs& a, void* scan_tmp, hipStream_t s)
+{
+    const int N = a.g.N;
     const size_t total = (size_t)a.grid.total;
     k_fill_u32<<<grid_for(total + 1), kBlock, 0, s>>>(a.cell_count, 0u, total + 1);
-    const unsigned g = grid_for(a.g.N);
-    RCP_DISPATCH_ND(a.g.ndim, k_cell_assign, <<<g, kBlock, 0, s>>>(a));
+    RCP_DISPATCH_ND(a.g.ndim, k_bin, <<<grid_for(N), kBlock, 0, s>>>(a));
     if (launch_scan_u32(a.cell_count, a.cell_start, total, scan_tmp, s)) return 1;
-    k_cell_scatter<<<g, kBlock, 0, s>>>(a);
+    k_place<<<grid_for(N), kBlock, 0, s>>>(a);
+    k_cell_order<<<grid_for(total), kBlock, 0, s>>>(a);
+    RCP_DISPATCH_ND(a.g.ndim, k_permute, <<<grid_for(N), kBlock, 0, s>>>(a));
     return check_launch();
 }
 
@@ -917,7 +1233,6 @@ int launch_rows_fill(ListArgs& a, hipStream_t s)
 
 size_t scan_tmp_bytes(size_t n)
 {
-    // per level: block sums (nb) + their scan (nb + 1), recursively
     size_t bytes = 0;
     size_t m = n;
     while (true) {
@@ -955,15 +1270,6 @@ size_t sort_hist_entries(size_t n)
     return 256 * nb + 1;
 }
 
-int launch_reorder_keys(const double* x, int N, int ndim, const double* box,
-                        unsigned long long* keys, uint32_t* vals, hipStream_t s)
-{
-    if (N <= 0) return 0;
-    const double b0 = box[0], b1 = ndim > 1 ? box[1] : 1.0, b2 = ndim > 2 ? box[2] : 1.0;
-    k_reorder_keys<<<grid_for(N), kBlock, 0, s>>>(x, N, ndim, b0, b1, b2, keys, vals);
-    return check_launch();
-}
-
 int launch_radix_sort(unsigned long long* keys, uint32_t* vals, size_t n, int key_bits,
                       SortTmp& t, hipStream_t s)
 {
@@ -974,7 +1280,7 @@ int launch_radix_sort(unsigned long long* keys, uint32_t* vals, size_t n, int ke
     uint32_t* vin = vals;
     uint32_t* vout = t.vals_alt;
     int passes = (key_bits + 7) / 8;
-    if (passes % 2) ++passes;  // even pass count: result lands back in keys/vals
+    if (passes % 2) ++passes;  // even pass count: the result lands back in keys/vals
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
         k_radix_hist<<<nb, kBlock, 0, s>>>(kin, n, shift, t.hist, nb);
@@ -986,31 +1292,85 @@ int launch_radix_sort(unsigned long long* keys, uint32_t* vals, size_t n, int ke
     return check_launch();
 }
 
-int launch_gather_f64(const double* src, double* dst, const uint32_t* perm, int N, int w,
+int launch_reference_reorder(const Geometry& g, const double* x, uint32_t* tag, uint32_t* inv_tag,
+                             uint32_t* tag_tmp, SortTmp& t, hipStream_t s)
+{
+    const int N = g.N;
+    if (N <= 0) return 0;
+    k_ref_keys<<<grid_for(N), kBlock, 0, s>>>(g, x, inv_tag, t.keys, t.vals);
+    if (launch_radix_sort(t.keys, t.vals, N, 64, t, s)) return 1;
+    k_tag_rank<<<grid_for(N), kBlock, 0, s>>>(t.vals, tag_tmp, N);
+    k_tag_apply<<<grid_for(N), kBlock, 0, s>>>(tag, inv_tag, tag_tmp, N);
+    return check_launch();
+}
+
+int launch_reorder_perm(const Geometry& g, const double* x_ref, uint32_t* perm, SortTmp& t,
+                        hipStream_t s)
+{
+    const int N = g.N;
+    if (N <= 0) return 0;
+    k_ref_keys<<<grid_for(N), kBlock, 0, s>>>(g, x_ref, nullptr, t.keys, perm);
+    if (launch_radix_sort(t.keys, perm, N, 64, t, s)) return 1;
+    return check_launch();
+}
+
+int launch_to_ref_f64(const double* src, double* dst, const uint32_t* tag, int N, int width,
                       hipStream_t s)
 {
     if (N <= 0) return 0;
-    k_gather_f64<<<grid_for((size_t)N * w), kBlock, 0, s>>>(src, dst, perm, N, w);
+    k_to_ref_f64<<<grid_for((size_t)N * width), kBlock, 0, s>>>(src, dst, tag, N, width);
     return check_launch();
 }
-int launch_gather_u64(const unsigned long long* src, unsigned long long* dst,
-                      const uint32_t* perm, int N, hipStream_t s)
+int launch_to_ref_u64(const unsigned long long* src, unsigned long long* dst, const uint32_t* tag,
+                      int N, hipStream_t s)
 {
     if (N <= 0) return 0;
-    k_gather_u64<<<grid_for(N), kBlock, 0, s>>>(src, dst, perm, N);
+    k_to_ref_u64<<<grid_for(N), kBlock, 0, s>>>(src, dst, tag, N);
     return check_launch();
 }
-int launch_gather_i8(const int8_t* src, int8_t* dst, const uint32_t* perm, int N,
-                     hipStream_t s)
+int launch_to_ref_i8(const int8_t* src, int8_t* dst, const uint32_t* tag, int N, hipStream_t s)
 {
     if (N <= 0) return 0;
-    k_gather_i8<<<grid_for(N), kBlock, 0, s>>>(src, dst, perm, N);
+    k_to_ref_i8<<<grid_for(N), kBlock, 0, s>>>(src, dst, tag, N);
+    return check_launch();
+}
+int launch_from_ref_f64(const double* src, double* dst, const uint32_t* tag, int N, int width,
+                        hipStream_t s)
+{
+    if (N <= 0) return 0;
+    k_from_ref_f64<<<grid_for((size_t)N * width), kBlock, 0, s>>>(src, dst, tag, N, width);
+    return check_launch();
+}
+int launch_from_ref_i8(const int8_t* src, int8_t* dst, const uint32_t* tag, int N, hipStream_t s)
+{
+    if (N <= 0) return 0;
+    k_from_ref_i8<<<grid_for(N), kBlock, 0, s>>>(src, dst, tag, N);
+    return check_launch();
+}
+int launch_rows_to_ref_count(const uint32_t* row_start, const uint32_t* tag, uint32_t* ref_count,
+                             int N, hipStream_t s)
+{
+    if (N <= 0) return 0;
+    k_rows_ref_count<<<grid_for(N), kBlock, 0, s>>>(row_start, tag, ref_count, N);
+    return check_launch();
+}
+int launch_rows_to_ref_fill(const uint32_t* row_start, const uint32_t* nbr, const uint32_t* tag,
+                            const uint32_t* ref_start, uint32_t* ref_nbr, int N, hipStream_t s)
+{
+    if (N <= 0) return 0;
+    k_rows_ref_fill<<<grid_for(N), kBlock, 0, s>>>(row_start, nbr, tag, ref_start, ref_nbr, N);
     return check_launch();
 }
 int launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s)
 {
     if (n == 0) return 0;
     k_fill_u32<<<grid_for(n), kBlock, 0, s>>>(p, v, n);
+    return check_launch();
+}
+int launch_iota_u32(uint32_t* p, size_t n, hipStream_t s)
+{
+    if (n == 0) return 0;
+    k_iota_u32<<<grid_for(n), kBlock, 0, s>>>(p, n);
     return check_launch();
 }
 int launch_iota_u64(unsigned long long* p, size_t n, hipStream_t s)
@@ -1032,6 +1392,13 @@ int launch_diameters(const double* D0, const int8_t* fixed, double kappa, double
     k_diameters<<<grid_for(N), kBlock, 0, s>>>(D0, fixed, kappa, D, N);
     return check_launch();
 }
+int launch_rk_active(const double* D0, const int8_t* fixed, double kappa, double* rk, int N,
+                     hipStream_t s)
+{
+    if (N <= 0) return 0;
+    k_rk_active<<<grid_for(N), kBlock, 0, s>>>(D0, fixed, kappa, rk, N);
+    return check_launch();
+}
 int launch_adam_pass(const Geometry& g, const double* F, double* m, double* v, const double* x,
                      double* xo, double alpha, double inv_b1, double inv_b2, hipStream_t s)
 {
@@ -1045,19 +1412,6 @@ int launch_mean_force(const StepArgs& a, const double* F, int exact, hipStream_t
     if (a.g.N <= 0) return 0;
     k_mean_force<<<grid_for(a.g.N), kBlock, 0, s>>>(a, F);
     if (exact) k_mean_force_exact<<<1, kWave, 0, s>>>(a);
-    return check_launch();
-}
-int launch_rk_active(const double* D0, const int8_t* fixed, double kappa, double* rk, int N,
-                     hipStream_t s)
-{
-    if (N <= 0) return 0;
-    k_rk_active<<<grid_for(N), kBlock, 0, s>>>(D0, fixed, kappa, rk, N);
-    return check_launch();
-}
-int launch_copy_f64(const double* src, double* dst, size_t n, hipStream_t s)
-{
-    if (n == 0) return 0;
-    k_copy_f64<<<grid_for(n), kBlock, 0, s>>>(src, dst, n);
     return check_launch();
 }
 

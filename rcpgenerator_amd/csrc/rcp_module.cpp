@@ -146,10 +146,13 @@ void parse_run(const py::dict& input, const py::dict& config, const py::dict& op
         c.initial_phi_target = options["initial_phi_target"].cast<double>();
     }
     bool exact = false, time_kernels = false;
+    int neighbor_mode = 0;
     take(options, "exact_reductions", exact);
     take(options, "time_kernels", time_kernels);
+    take(options, "neighbor_mode", neighbor_mode);
     c.exact_reductions = exact;
     c.time_kernels = time_kernels;
+    c.neighbor_mode = neighbor_mode;
 }
 
 py::dict result_dict(rcp_engine* e)
@@ -333,8 +336,8 @@ public:
         rcp_result_info i;
         int rc = rcp_engine_info(h_.e, &i);
         if (rc) raise_rc(rc);
-        uint64_t builds = 0, entries = 0, reorders = 0;
-        rcp_engine_stats(h_.e, &builds, &entries, &reorders);
+        uint64_t builds = 0, entries = 0, reorders = 0, direct = 0, overflow = 0;
+        rcp_engine_stats(h_.e, &builds, &entries, &reorders, &direct, &overflow);
         py::dict d;
         d["N"] = i.N;
         d["ndim"] = i.ndim;
@@ -346,6 +349,8 @@ public:
         d["list_builds"] = builds;
         d["list_entries"] = entries;
         d["reorders"] = reorders;
+        d["direct_steps"] = direct;
+        d["overflow_steps"] = overflow;
         double sm = 0, bm = 0;
         uint64_t sn = 0, bn = 0;
         rcp_engine_timing(h_.e, &sm, &sn, &bm, &bn);
