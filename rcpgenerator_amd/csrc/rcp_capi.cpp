@@ -380,7 +380,9 @@ int rcp_pairs(size_t N, size_t ndim, const double* x, const double* D, const dou
         const double t_shell = rcp::reference_shell(Dv);
         const double Dmax = *std::max_element(D, D + N);
         S.rebin(g, Dmax + t_shell);
-        S.build_rows(g, 1.0, t_shell, 1, 0);
+        // ndim > 3: the sort+walk builder's cut-off with its float half
+        // diameter (rcp_generator.cpp:1357); its wrap duplicates are not kept
+        S.build_rows(g, 1.0, t_shell, 1, ndim > 3 ? 1 : 0);
         *npairs = S.nnz;
         // rows re-indexed to the input order: owner by tag, partners as tags
         uint32_t* ref_count = S.alloc<uint32_t>(N + 1);

@@ -1,0 +1,400 @@
+"""GPU parity: the HIP engine through the C ABI against the oracle and the
+golden vectors written by the reference build.
+
+Bar: bit-for-bit for forces, pair rows, reorder permutations, ADAM, the mean
+force, and every output of a run in exact-reduction mode (the reference's
+one-thread summation order).  The default fast mode folds the scalar sums in
+block order; per evaluation its scalars agree with the reference order to
+1e-12 relative (forces stay bit-for-bit), and whole runs agree statistically.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HIST = ("phi_history", "force_history", "energy_history", "max_overlap_history",
+        "mu_history", "alpha_history")
+SCALARS = ("U", "max_min_dist", "Lc", "Fmean", "dkappa")
+
+
+def bits(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a.view(np.uint64)
+
+
+def same(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return a.shape == b.shape and np.array_equal(bits(a), bits(b))
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from rcpgenerator_amd import capi
+    info = capi.device_info()
+    assert "gfx950" in info, info
+    return capi
+
+
+def random_config(rng, N, nd, phi, poly=0.2, walls=None):
+    D = np.exp(rng.normal(0.0, poly, N))
+    unit = math.pi ** (nd / 2) / math.gamma(nd / 2 + 1)
+    D *= (phi / (unit * np.sum((D / 2) ** nd))) ** (1.0 / nd)
+    x = rng.random((N, nd))
+    if walls is not None and walls[0] < 0:
+        t = -walls[0]
+        x[:, :t] = 0.5 + (x[:, :t] - 0.5) * 0.8
+    return x, D, np.ones(nd)
+
+
+# ------------------------------------------------------------------ forces
+
+def test_forces_golden(gpu, golden):
+    g = golden("forces.npz")
+    for k in range(int(g["ncases"])):
+        x, D, box, walls, mu = (g[f"c{k}_{n}"] for n in ("x", "D", "box", "walls", "mu"))
+        F, s = gpu.forces(x, D, box, walls, mu=float(mu), exact=True)
+        assert same(F, g[f"c{k}_F"]), f"case {k}: forces"
+        got = np.array([s[n] for n in SCALARS])
+        assert same(got, g[f"c{k}_scalars"]), f"case {k}: scalars {got} vs {g[f'c{k}_scalars']}"
+        Ff, sf = gpu.forces(x, D, box, walls, mu=float(mu), exact=False)
+        assert same(Ff, g[f"c{k}_F"]), f"case {k}: fast-mode forces"
+        for n, want in zip(SCALARS, g[f"c{k}_scalars"]):
+            assert abs(sf[n] - want) <= 1e-12 * abs(want) + 1e-300, (k, n)
+
+
+FORCE_CASES = [
+    (1, [0], 300, 0.9), (1, [1], 300, 0.9),
+    (2, [0, 0], 4000, 0.6), (2, [0, 0], 4000, 1.3), (2, [1, 1], 3000, 0.8),
+    (2, [-2, -1], 2000, 0.6),
+    (3, [0, 0, 0], 4000, 0.6), (3, [0, 0, 1], 4000, 0.9), (3, [-3, -1, -1], 2000, 0.5),
+    (3, [-2, -1, 0], 2000, 0.5),
+    (4, [0, 0, 0, 0], 2000, 0.4), (5, [0, 0, 0, 1, 1], 1500, 0.3),
+    (6, [0] * 6, 800, 0.2), (8, [0] * 8, 600, 0.1),
+]
+
+
+@pytest.mark.parametrize("nd,walls,N,phi", FORCE_CASES)
+def test_forces_vs_oracle(gpu, oracle, nd, walls, N, phi):
+    rng = np.random.default_rng(1000 * nd + N + int(10 * phi))
+    x, D, box = random_config(rng, N, nd, phi, walls=walls)
+    w = np.array(walls, dtype=np.int8)
+    Fo, so = oracle.forces(x, D, box, w, 3e-4)
+    F, s = gpu.forces(x, D, box, w, mu=3e-4, exact=True)
+    assert same(F, Fo)
+    for n in SCALARS:
+        assert same(s[n], so[n]), n
+    assert s["count"] == so["count"]
+
+
+def test_forces_cascade_weights(gpu, oracle):
+    """Explicit rk (frozen particles weigh 0) and an explicit active sum(D)."""
+    rng = np.random.default_rng(77)
+    x, D, box = random_config(rng, 3000, 2, 0.9)
+    rk = np.where(rng.random(3000) < 0.3, 0.0, D / 2)
+    sumD = float(np.sum(D[rk > 0]))
+    w = np.zeros(2, dtype=np.int8)
+    Fo, so = oracle.forces(x, D, box, w, 2.5e-4, rk=rk, sumD_active=sumD)
+    F, s = gpu.forces(x, D, box, w, mu=2.5e-4, rk=rk, sumD_active=sumD, exact=True)
+    assert same(F, Fo)
+    for n in SCALARS:
+        assert same(s[n], so[n]), n
+
+
+def test_forces_edge_cases(gpu, oracle):
+    # a single particle against the walls, a pair overlapping through the
+    # periodic boundary, a sparse system with no contacts, coordinates at 0
+    cases = [
+        (np.array([[0.01, 0.5]]), np.array([0.1]), [1, 1]),
+        (np.array([[0.005, 0.5], [0.995, 0.5]]), np.array([0.03, 0.03]), [0, 0]),
+        (np.random.default_rng(4).random((500, 3)), np.full(500, 1e-4), [0, 0, 0]),
+        (np.array([[0.0, 0.0], [0.02, 0.0], [0.0, 0.99]]), np.array([0.05, 0.05, 0.05]), [0, 0]),
+    ]
+    for x, D, walls in cases:
+        nd = x.shape[1]
+        w = np.array(walls, dtype=np.int8)
+        Fo, so = oracle.forces(x, D, np.ones(nd), w, 2.5e-4)
+        F, s = gpu.forces(x, D, np.ones(nd), w, mu=2.5e-4, exact=True)
+        assert same(F, Fo)
+        for n in SCALARS:
+            assert same(s[n], so[n]), n
+
+
+def test_forces_full_size(gpu, oracle):
+    """Bench size: 10^6 disks at the engine's start density (phi 0.575)."""
+    rng = np.random.default_rng(123)
+    x, D, box = random_config(rng, 1_000_000, 2, 0.575)
+    w = np.zeros(2, dtype=np.int8)
+    Fo, so = oracle.forces(x, D, box, w, 2.5e-4)
+    F, s = gpu.forces(x, D, box, w, mu=2.5e-4, exact=True)
+    assert same(F, Fo)
+    for n in SCALARS:
+        assert same(s[n], so[n]), n
+    Ff, sf = gpu.forces(x, D, box, w, mu=2.5e-4, exact=False)
+    assert same(Ff, Fo)
+    for n in ("U", "Lc", "Fmean", "dkappa"):
+        assert abs(sf[n] - so[n]) <= 1e-10 * abs(so[n]), n
+    assert sf["max_min_dist"] == so["max_min_dist"]
+
+
+# ------------------------------------------------------------------ pairs
+
+def test_pairs_golden(gpu, golden):
+    g = golden("pairs.npz")
+    for k in range(int(g["ncases"])):
+        x, D = g[f"p{k}_x"], g[f"p{k}_D"]
+        nd = x.shape[1]
+        off, nbr = gpu.pairs(x, D, np.ones(nd))
+        if nd <= 3:
+            assert np.array_equal(off, g[f"p{k}_off"]) and np.array_equal(nbr, g[f"p{k}_nbr"])
+        else:
+            # sort+walk rows: same pair set; the walk's wrap duplicates are
+            # not reproduced
+            want = set()
+            o = g[f"p{k}_off"]
+            for i in range(len(D)):
+                want.update((i, int(j)) for j in g[f"p{k}_nbr"][o[i]:o[i + 1]])
+            got = {(i, int(j)) for i in range(len(D)) for j in nbr[off[i]:off[i + 1]]}
+            assert got == want
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_pairs_vs_oracle(gpu, oracle, nd):
+    rng = np.random.default_rng(31 + nd)
+    x, D, box = random_config(rng, 20000, nd, 0.5)
+    off, nbr = gpu.pairs(x, D, box)
+    oo, on = oracle.pairs_full(x, D, box)
+    assert np.array_equal(off, oo) and np.array_equal(nbr, on)
+
+
+# ----------------------------------------------------- reorder, ADAM, mean
+
+@pytest.mark.parametrize("nd", [2, 3, 4])
+def test_reorder_perm(gpu, oracle, nd):
+    rng = np.random.default_rng(nd)
+    x = rng.random((50000, nd))
+    x[::5, 0] = np.round(x[::5, 0], 2)  # ties: the sort must be stable
+    x[::7] = np.round(x[::7], 3)
+    x[3, 0] = -0.0
+    x[4, 0] = 0.0
+    box = np.ones(nd)
+    assert np.array_equal(gpu.reorder_perm(x, box), oracle.reorder_perm(x, box))
+
+
+def test_adam_and_mean_force(gpu, oracle):
+    rng = np.random.default_rng(9)
+    F = rng.normal(size=(5000, 3))
+    F[::9] = 0.0
+    m = rng.normal(size=F.shape) * 1e-3
+    v = rng.random(F.shape) * 1e-4
+    x = rng.random(F.shape)
+    x[0] = [0.0, 1.0 - 1e-17, 0.999999]
+    args = (np.ones(3), 0.004, 1 / (1 - 0.9 ** 7), 1 / (1 - 0.999 ** 7))
+    for a, b in zip(gpu.adam_positions(F, m, v, x, *args), oracle.adam_positions(F, m, v, x, *args)):
+        assert same(a, b)
+    assert gpu.mean_force(F) == oracle.mean_force(F)
+
+
+# ------------------------------------------------------------------- runs
+
+def engine_run(gpu, x, D, **kw):
+    e = gpu.Engine(x, D, exact=True, **kw)
+    e.run(0)
+    e.finish()
+    return e.result(), e.stats()
+
+
+def test_runs_golden(gpu, golden):
+    import hashlib
+    g = golden("runs.npz")
+    for name in g["names"]:
+        name = str(name)
+        ms = int(g[f"{name}_max_steps"])
+        r, _ = engine_run(gpu, g[f"{name}_x"], g[f"{name}_D"], box=g[f"{name}_box"],
+                          walls=g[f"{name}_walls"], max_steps=ms)
+        assert r["steps"] == int(g[f"{name}_steps"]), name
+        assert r["phi"] == float(g[f"{name}_phi"]), name
+        assert same(r["positions"], g[f"{name}_final_x"]), name
+        assert same(r["diameters"], g[f"{name}_final_D"]), name
+        assert np.array_equal(r["particle_origin"], g[f"{name}_origin"]), name
+        for key in HIST:
+            dig = hashlib.sha256(np.ascontiguousarray(r[key]).tobytes()).hexdigest()
+            assert dig == str(g[f"{name}_{key}_sha256"]), (name, key)
+        dig = hashlib.sha256(np.ascontiguousarray(r["mu_flag_history"]).tobytes()).hexdigest()
+        assert dig == str(g[f"{name}_mu_flag_history_sha256"]), name
+
+
+RUNS = [
+    (60, 2, [0, 0], {"type": "bidisperse", "d1": 1.0, "d2": 1.4, "p": 0.5}, {}),
+    (60, 3, [0, 0, 0], {"type": "lognormal", "mu": 0.0, "sigma": 0.2}, {}),
+    (50, 3, [1, 1, 1], {"type": "mono", "d": 1.0}, {"max_steps": 3000}),
+    (60, 2, [-1, -1], {"type": "powerlaw", "d_min": 0.3, "d_max": 1.0, "exponent": -3.0}, {}),
+    (40, 4, [0, 0, 0, 0], {"type": "mono", "d": 1.0}, {"max_steps": 4000}),
+    (80, 2, [0, 0], {"type": "mono", "d": 1.0}, {"max_steps": 2500, "mu": 1e-3,
+                                                 "target_phi": 0.7}),
+    (80, 2, [0, 1], {"type": "flat", "d_min": 0.5, "d_max": 1.5},
+     {"max_steps": 1200, "reorder_interval": 300, "initial_phi_target": 0.6}),
+    (80, 2, [0, 0], {"type": "mono", "d": 1.0}, {"max_steps": 900, "fix_diameter": True}),
+    (400, 3, [0, 0, 1], {"type": "bidisperse", "d1": 1.0, "d2": 1.4, "p": 0.5},
+     {"max_steps": 1500, "reorder_interval": 7}),
+    (300, 5, [0, 0, 0, 0, 0], {"type": "lognormal", "mu": 0.0, "sigma": 0.1},
+     {"max_steps": 800}),
+]
+
+
+@pytest.mark.parametrize("N,nd,walls,dist,opts", RUNS)
+def test_run_vs_oracle(gpu, oracle, N, nd, walls, dist, opts):
+    import rcpgenerator_amd as rcp
+    init = rcp.initialize_particles({"phi": 0.1, "N": N, "Ndim": nd, "box": [1.0] * nd,
+                                     "walls": walls, "seed": 5, "dist": dist})
+    kw = dict(box=init["box"], walls=init["walls"], max_steps=opts.get("max_steps", 0),
+              mu=opts.get("mu"), fix_diameter=opts.get("fix_diameter", False),
+              target_phi=opts.get("target_phi"),
+              reorder_interval=opts.get("reorder_interval", 2000),
+              initial_phi_target=opts.get("initial_phi_target"))
+    r, st = engine_run(gpu, init["positions"], init["diameters"], **kw)
+    o = oracle.run_packing(init["positions"], init["diameters"], **kw)
+    assert r["steps"] == o["steps"]
+    assert same(r["positions"], o["positions"])
+    assert same(r["diameters"], o["diameters"])
+    assert np.array_equal(r["particle_origin"], o["particle_origin"])
+    for key in HIST:
+        assert same(r[key], o[key]), key
+    assert np.array_equal(r["mu_flag_history"], o["mu_flag_history"])
+    assert r["phi"] == o["phi"] and r["max_min_dist"] == o["max_min_dist"]
+    if opts.get("reorder_interval") == 7:
+        assert st["reorders"] > 100
+
+
+def test_cascade_run_vs_oracle(gpu, oracle):
+    import rcpgenerator_amd as rcp
+    init = rcp.initialize_particles({"phi": 0.1, "N": 60, "Ndim": 2, "box": [1.0, 1.0],
+                                     "walls": [0, 0], "seed": 2,
+                                     "dist": {"type": "bidisperse", "d1": 1.0, "d2": 2.0,
+                                              "p": 0.5}})
+    D = np.array(init["diameters"])
+    fixed = (D > 0.9 * D.max()).astype(np.int8)
+    kw = dict(box=[1.0, 1.0], walls=[0, 0], fixed=fixed, max_steps=7000)
+    r, _ = engine_run(gpu, init["positions"], D, **kw)
+    o = oracle.run_packing(init["positions"], D, **kw)
+    assert r["steps"] == o["steps"]
+    assert same(r["positions"], o["positions"]) and same(r["diameters"], o["diameters"])
+    for key in HIST:
+        assert same(r[key], o[key]), key
+
+
+@pytest.mark.parametrize("max_steps", [50, 300])
+def test_fix_height_run_vs_oracle(gpu, oracle, max_steps):
+    import rcpgenerator_amd as rcp
+    init = rcp.initialize_particles({"phi": 0.1, "N": 60, "Ndim": 2, "box": [1.0, 1.0],
+                                     "walls": [0, 1], "seed": 9, "fix_height": True,
+                                     "dist": {"type": "flat", "d_min": 0.5, "d_max": 1.0}})
+    kw = dict(box=init["box"], walls=init["walls"], fix_height=True, max_steps=max_steps)
+    r, _ = engine_run(gpu, init["positions"], init["diameters"], **kw)
+    o = oracle.run_packing(init["positions"], init["diameters"], **kw)
+    assert r["steps"] == o["steps"]
+    assert same(r["positions"], o["positions"]) and same(r["box"], o["box"])
+    for key in HIST:
+        assert same(r[key], o[key]), key
+
+
+# ------------------------------------------------- neighbour-mode agreement
+
+@pytest.mark.parametrize("nd,phi", [(2, 1.2), (3, 0.9), (2, 9.0)])
+def test_cell_scan_matches_lists(gpu, nd, phi):
+    """With the diameters fixed no scalar feeds back into the positions, so
+    the per-step cell scan and the neighbour lists must give bit-identical
+    trajectories; phi 9 in 2-D overflows the scan's contact buffer."""
+    rng = np.random.default_rng(int(10 * phi) + nd)
+    x, D, box = random_config(rng, 20000 if phi < 5 else 3000, nd, phi)
+    out = {}
+    for mode in (1, 2):
+        e = gpu.Engine(x, D, box=box, max_steps=40, fix_diameter=True, neighbor_mode=mode)
+        e.run(0)
+        out[mode] = (e.result(), e.stats())
+    (a, sa), (b, sb) = out[1], out[2]
+    assert sb["direct_steps"] == 40 and sa["direct_steps"] == 0
+    assert same(a["positions"], b["positions"])
+    if phi > 5:
+        assert sb["overflow_steps"] > 0
+
+
+def test_auto_mode_uses_both_paths(gpu):
+    import rcpgenerator_amd as rcp
+    init = rcp.initialize_particles({"phi": 0.1, "N": 50000, "Ndim": 2, "box": [1.0, 1.0],
+                                     "walls": [0, 0], "seed": 123,
+                                     "dist": {"type": "lognormal", "mu": -0.5 * math.log(1.04),
+                                              "sigma": math.sqrt(math.log(1.04))}})
+    e = gpu.Engine(init["positions"], init["diameters"], box=init["box"], walls=init["walls"],
+                   max_steps=6000)
+    e.run(0)
+    r, st = e.result(), e.stats()
+    assert st["direct_steps"] > 0 and st["list_builds"] > 1
+    assert np.all(np.isfinite(r["positions"]))
+    assert np.all((r["positions"] >= 0) & (r["positions"] < 1))
+    assert np.all(np.isfinite(r["phi_history"]))
+
+
+# ----------------------------------------------------------- large N
+
+def test_full_size_run_properties(gpu):
+    """10^6 disks, 60 steps of the bench workload: finite, wrapped, and the
+    kernel-level forces of the final state agree with a fresh evaluation."""
+    import rcpgenerator_amd as rcp
+    init = rcp.initialize_particles({"phi": 0.1, "N": 1_000_000, "Ndim": 2, "box": [1.0, 1.0],
+                                     "walls": [0, 0], "seed": 123,
+                                     "dist": {"type": "lognormal", "mu": -0.5 * math.log(1.04),
+                                              "sigma": math.sqrt(math.log(1.04))}})
+    e = gpu.Engine(init["positions"], init["diameters"], box=init["box"], walls=init["walls"],
+                   max_steps=60)
+    e.run(0)
+    r = e.result()
+    assert r["steps"] == 60
+    x = r["positions"]
+    assert np.all(np.isfinite(x)) and np.all((x >= 0) & (x < 1))
+    assert np.all(np.isfinite(r["phi_history"])) and np.all(np.isfinite(r["force_history"]))
+    # particle_origin is a permutation
+    assert np.array_equal(np.sort(r["particle_origin"]), np.arange(1_000_000, dtype=np.uint64))
+
+
+# --------------------------------------------------------- API end to end
+
+def test_packing_api(gpu):
+    import rcpgenerator_amd as rcp
+    p = rcp.Packing(phi=0.08, N=200, Ndim=3, box=[1.0, 1.0, 1.0], walls=[0, 0, 0],
+                    dist={"type": "bidisperse", "d1": 1.0, "d2": 1.4, "p": 0.5}, seed=123)
+    res = p.pack()
+    for key in ("positions", "diameters", "box", "walls", "phi", "steps", "max_min_dist",
+                "phi_history", "force_history", "energy_history", "particle_origin"):
+        assert key in res, key
+    assert len(res["positions"]) == 200 and res["steps"] > 0
+    assert 0.55 < res["phi"] < 0.70
+    assert res["max_min_dist"] < 1e-3
+    assert p.packed and p.steps == res["steps"]
+    # relax() continues from the realized state
+    r2 = p.relax(200, fix_diameter=True)
+    assert r2["steps"] == 200
+
+
+def test_observed_trajectory_matches_stepwise(gpu):
+    import rcpgenerator_amd as rcp
+    init = rcp.initialize_particles({"phi": 0.1, "N": 300, "Ndim": 2, "box": [1.0, 1.0],
+                                     "walls": [1, 0], "seed": 3,
+                                     "dist": {"type": "mono", "d": 1.0}})
+    inp = {"positions": init["positions"], "diameters": init["diameters"]}
+    cfg = {"box": init["box"], "walls": init["walls"]}
+    seen = []
+    out = rcp._rcpgenerator.run_packing_observed(
+        inp, cfg, 250, True, 500, lambda u: seen.append(u["step"]),
+        {"max_steps": 2000, "exact_reductions": True})
+    steps = [int(s) for s in out["trace"]["steps"]]
+    assert steps[:5] == [0, 500, 1000, 1500, 2000]
+    assert seen[:3] == [0, 250, 500]
+    e = gpu.Engine(init["positions"], init["diameters"], box=init["box"], walls=init["walls"],
+                   max_steps=2000, exact=True)
+    e.run(1000)
+    k = steps.index(1000)
+    assert same(np.array(out["trace"]["positions"][k]), e.result()["positions"])

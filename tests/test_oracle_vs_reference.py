@@ -122,6 +122,25 @@ def test_full_run_bit_exact(ref, N, nd, walls, dist, opts):
     assert r["phi"] == o["phi"] and r["max_min_dist"] == o["max_min_dist"]
 
 
+@pytest.mark.parametrize("max_steps", [50, 300])
+def test_fix_height_run_bit_exact(ref, max_steps):
+    """fix_height (the box height follows kappa every step, RG:5773-5779)."""
+    mod = ref.ref_module()
+    mod.set_num_threads(1)
+    init = mod.initialize_particles({"phi": 0.1, "N": 60, "Ndim": 2, "box": [1.0, 1.0],
+                                     "walls": [0, 1], "seed": 9, "fix_height": True,
+                                     "dist": {"type": "flat", "d_min": 0.5, "d_max": 1.0}})
+    r = mod.run_packing_observed({"positions": init["positions"], "diameters": init["diameters"]},
+                                 {"box": init["box"], "walls": init["walls"], "fix_height": True},
+                                 options={"max_steps": max_steps})["result"]
+    o = ref.run_packing(init["positions"], init["diameters"], box=init["box"],
+                        walls=init["walls"], fix_height=True, max_steps=max_steps)
+    assert r["steps"] == o["steps"]
+    assert np.array_equal(_bits(np.array(r["positions"])), _bits(o["positions"]))
+    assert np.array_equal(_bits(np.array(r["box"])), _bits(o["box"]))
+    assert np.array_equal(_bits(np.array(r["phi_history"])), _bits(o["phi_history"]))
+
+
 def test_cascade_run_bit_exact(ref):
     """The frozen-diameter mask (PackingInput.fixed, rcp_generator.hpp:30-32)."""
     mod = ref.ref_module()
