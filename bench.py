@@ -12,14 +12,24 @@ value = N * steps / wall-seconds of the timed region.
 The timed region runs steps 1..K of a packing created (initializer, upload,
 initial reorder, first list build) before the clock starts, with inputs
 resident in HBM.  W warm-up steps run first on a separate engine built from
-the same initial state.  The dominant kernel (the fused step kernel) is
-timed live with HIP events on the engine's own stream; its algorithmic HBM
-bytes are A(2) = 8*(6*2+1) = 104 B per particle (x read+write, ADAM m and v
-read+write, D0 read) times N per launch.
+the same initial state.  The dominant kernel (the fused step kernel: the
+LDS-tiled cell-stencil kernel k_step_tile<2,128>, whose cell binning is
+refreshed every step while particles move fast and reused under a skin
+later) is timed live with HIP events on the engine's own stream; its
+algorithmic HBM bytes are A(2) = 8*(6*2+1) = 104 B per particle (x
+read+write, ADAM m and v read+write, D0 read) times N per launch.
 
 The CPU baseline runs the reference engine (built from its sources into
-oracle/_ref; AVX-512 build when the host supports it) with all host cores on
-the first steps of the same packing.
+oracle/_ref; AVX-512 build when the host supports it) on this process's
+share of the host cores, on the first steps of the same packing.
+
+Profiling helpers (rocprofv3 runs of this same file):
+  --copy-calibration    before the engine, one device-to-device copy of
+                        256 MiB, the known-size kernel the FETCH_SIZE and
+                        WRITE_SIZE counters are scaled by
+  --pmc-summary F W     read rocprofv3 --pmc counter_collection.csv files of
+                        a FETCH_SIZE run (F) and a WRITE_SIZE run (W) and
+                        print HBM bytes per launch of the step kernel
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -51,6 +61,19 @@ def workload(N: int):
     return init
 
 
+def host_cores() -> int:
+    """Cores this process may use: its affinity mask, capped by
+    OMP_NUM_THREADS when that is set (the share of a shared host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(n, 1)
+
+
 def has_avx512() -> bool:
     try:
         with open("/proc/cpuinfo") as f:
@@ -69,7 +92,7 @@ def cpu_baseline(inp, cfg, N, budget_s: float):
     native = has_avx512() and os.path.exists(
         os.path.join(O.REF_DIR, "native", "_rcpgenerator" + O.EXT_SUFFIX))
     mod = O.ref_module(native=native)
-    cores = os.cpu_count() or 1
+    cores = host_cores()
     mod.set_num_threads(cores)
 
     def timed(k):
@@ -90,15 +113,73 @@ def cpu_baseline(inp, cfg, N, budget_s: float):
                       f"build, set_num_threads({cores})"}
 
 
+COPY_BYTES = 256 << 20
+
+
+def copy_calibration():
+    """One device-to-device copy of COPY_BYTES (the runtime's copy kernel)."""
+    import torch
+    a = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+
+
+def _pmc_rows(path):
+    import csv
+    with open(path, newline="") as f:
+        for row in csv.DictReader(f):
+            yield row.get("Kernel_Name", ""), row.get("Counter_Name", ""), \
+                float(row.get("Counter_Value", "0") or 0)
+
+
+def pmc_summary(fetch_csv, write_csv):
+    """HBM bytes per launch of the step kernels from two rocprofv3 --pmc
+    runs, each counter scaled by its reading on the copy of COPY_BYTES."""
+    out = {"copy_bytes": COPY_BYTES}
+    for label, path, counter in (("fetch", fetch_csv, "FETCH_SIZE"),
+                                 ("write", write_csv, "WRITE_SIZE")):
+        per = {}
+        for name, cname, val in _pmc_rows(path):
+            if not cname.startswith(counter):
+                continue
+            if "copyBuffer" in name or "copy" in name.lower() and "rcp" not in name:
+                key = "copy"
+            elif "k_step_tile" in name:
+                key = "k_step_tile"
+            elif "k_step_direct" in name:
+                key = "k_step_direct"
+            elif "k_step_list" in name:
+                key = "k_step_list"
+            else:
+                continue
+            per.setdefault(key, []).append(val * 1024.0)  # KiB per dispatch
+        copy = max(per.get("copy", [0.0]))
+        factor = COPY_BYTES / copy if copy > 0 else None
+        out[f"{label}_copy_counter_bytes"] = copy
+        out[f"{label}_scale"] = factor
+        for key in ("k_step_tile", "k_step_direct", "k_step_list"):
+            v = per.get(key)
+            if v and factor:
+                out[f"{key}_{label}_bytes"] = factor * sum(v) / len(v)
+                out[f"{key}_launches"] = len(v)
+    for key in ("k_step_tile", "k_step_direct", "k_step_list"):
+        if f"{key}_fetch_bytes" in out and f"{key}_write_bytes" in out:
+            out[f"{key}_bytes_per_launch"] = out[f"{key}_fetch_bytes"] + out[f"{key}_write_bytes"]
+    return out
+
+
 def read_traffic():
     """HBM bytes per launch of the fused step kernel from the committed
     rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled)."""
     p = os.path.join(ROOT, "profiles", "k_step_2d_1M_traffic.json")
     try:
         with open(p) as f:
-            return json.load(f).get("bytes_per_launch")
+            d = json.load(f)
     except (OSError, ValueError):
         return None
+    return d.get("k_step_tile_bytes_per_launch")
 
 
 def main() -> int:
@@ -109,12 +190,19 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     args = ap.parse_args()
+    if args.pmc_summary:
+        print(json.dumps(pmc_summary(*args.pmc_summary), indent=1))
+        return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs one packing per GPU and this build measures one "
               "GPU only; run with --gpus 1", file=sys.stderr)
         return 2
 
+    if args.copy_calibration:
+        copy_calibration()
     import rcpgenerator_amd as rcp
     N, K, W = args.n, args.steps, args.warmup
     log(f"[bench] device: {rcp._rcpgenerator.device_info()}")
@@ -139,18 +227,21 @@ def main() -> int:
     eng.sync()
     dt = time.perf_counter() - t0
     info = eng.info()
-    log(f"[bench] {done} steps in {dt:.3f}s; list builds {info['list_builds']}, "
+    tiles = info["tile_steps"] - base["tile_steps"]
+    rebins = info["tile_rebins"] - base["tile_rebins"]
+    log(f"[bench] {done} steps in {dt:.3f}s; tiled steps {tiles}, re-binnings {rebins}, "
         f"reorders {info['reorders']}")
     value = N * done / dt
     launches = info["step_kernel_launches"] - base["step_kernel_launches"]
     kms = (info["step_kernel_ms"] - base["step_kernel_ms"]) / max(launches, 1)
     algo_bytes = 8 * (6 * 2 + 1) * N
     achieved = algo_bytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
-    roofline = {"bound": "hbm", "kernel": "k_step<2> (fused forces+ADAM)",
+    kname = "k_step_tile<2,128> (fused forces + walls + reductions + ADAM)"
+    roofline = {"bound": "hbm", "kernel": kname,
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
                 "avg_launch_ms": round(kms, 5), "algorithmic_bytes_per_launch": algo_bytes,
-                "list_build_ms_per_step": round(
+                "rebin_ms_per_step": round(
                     (info["build_ms"] - base["build_ms"]) / max(done, 1), 5)}
     cpu = None
     if not args.no_cpu:

@@ -134,13 +134,15 @@ int rcp_engine_trace(const rcp_engine* e, size_t k, size_t* step, double* phi,
                      double* force, double* energy, double* max_min_dist,
                      double* positions, double* diameters, int* has_positions);
 
-/* Number of neighbour-list builds and total list entries so far, reference
- * reorders, steps taken without a list (per-step cell scan) and steps in
- * which some particle had more contacts than the scan's buffer (those take
- * the slower selection path).  Diagnostics only. */
+/* Diagnostics: neighbour-list builds and total list entries so far,
+ * reference reorders, steps taken by the per-step cell scan, steps in which
+ * some particle had more contacts than a scan's buffer (those take the
+ * slower selection path), steps taken by the tiled kernel and the
+ * re-binnings that served them. */
 int rcp_engine_stats(const rcp_engine* e, uint64_t* list_builds,
                      uint64_t* list_entries, uint64_t* reorders,
-                     uint64_t* direct_steps, uint64_t* overflow_steps);
+                     uint64_t* direct_steps, uint64_t* overflow_steps,
+                     uint64_t* tile_steps, uint64_t* tile_rebins);
 
 /* Accumulated HIP-event durations (time_kernels = 1): fused-step kernel
  * launches and neighbour-list builds, in milliseconds.  No reference

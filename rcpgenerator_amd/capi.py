@@ -87,7 +87,7 @@ def lib():
         L.rcp_engine_sync.argtypes = [C.c_void_p]
         L.rcp_engine_info.argtypes = [C.c_void_p, C.POINTER(ResultInfo)]
         L.rcp_engine_result.argtypes = [C.c_void_p] + [C.c_void_p] * 12
-        L.rcp_engine_stats.argtypes = [C.c_void_p] + [C.POINTER(C.c_uint64)] * 5
+        L.rcp_engine_stats.argtypes = [C.c_void_p] + [C.POINTER(C.c_uint64)] * 7
         L.rcp_engine_timing.argtypes = [C.c_void_p, C.POINTER(C.c_double),
                                         C.POINTER(C.c_uint64), C.POINTER(C.c_double),
                                         C.POINTER(C.c_uint64)]
@@ -239,11 +239,11 @@ class Engine:
         return i
 
     def stats(self):
-        b, e, r, d, o = (C.c_uint64() for _ in range(5))
-        _check(lib().rcp_engine_stats(self._h, C.byref(b), C.byref(e), C.byref(r), C.byref(d),
-                                      C.byref(o)))
-        return {"list_builds": b.value, "list_entries": e.value, "reorders": r.value,
-                "direct_steps": d.value, "overflow_steps": o.value}
+        v = [C.c_uint64() for _ in range(7)]
+        _check(lib().rcp_engine_stats(self._h, *[C.byref(x) for x in v]))
+        keys = ("list_builds", "list_entries", "reorders", "direct_steps", "overflow_steps",
+                "tile_steps", "tile_rebins")
+        return {k: x.value for k, x in zip(keys, v)}
 
     def timing(self):
         sm, bm = C.c_double(), C.c_double()

@@ -239,7 +239,8 @@ int rcp_engine_trace(const rcp_engine* e, size_t k, size_t* step, double* phi, d
 }
 
 int rcp_engine_stats(const rcp_engine* e, uint64_t* list_builds, uint64_t* list_entries,
-                     uint64_t* reorders, uint64_t* direct_steps, uint64_t* overflow_steps)
+                     uint64_t* reorders, uint64_t* direct_steps, uint64_t* overflow_steps,
+                     uint64_t* tile_steps, uint64_t* tile_rebins)
 {
     return guarded([&]() {
         if (!e) throw rcp::ArgError("rcp: null engine");
@@ -248,6 +249,8 @@ int rcp_engine_stats(const rcp_engine* e, uint64_t* list_builds, uint64_t* list_
         if (reorders) *reorders = e->e->reorders();
         if (direct_steps) *direct_steps = e->e->direct_steps();
         if (overflow_steps) *overflow_steps = e->e->overflow_steps();
+        if (tile_steps) *tile_steps = e->e->tile_steps();
+        if (tile_rebins) *tile_rebins = e->e->tile_rebins();
         return RCP_OK;
     });
 }
@@ -340,9 +343,10 @@ int rcp_forces(size_t N, size_t ndim, const double* x, const double* D, const do
         a.F_out = S.anchor;
         a.inv_tag = S.inv_tag;
         a.partials = S.partials;
-        a.counter = S.counter;
         a.result = S.result;
         if (exact) {
+            S.ensure_inv_tag();
+            a.inv_tag = S.inv_tag;
             a.mag_out = S.mag;
             a.contrib = S.contrib;
             if (rcp::launch_exact_step(a, S.ref_count, S.ref_start, S.ref_terms, S.scan_tmp, S.s))
@@ -416,14 +420,11 @@ int rcp_mean_force(size_t N, size_t ndim, const double* F, double* out)
         g.ndim = (int)ndim;
         DBuf<double> dF(N * ndim), dmag(N), dpart((rcp::step_partial_blocks(N) + 1) * rcp::S_NUM),
             dres(rcp::S_NUM);
-        DBuf<unsigned int> dcnt(1);
         dF.up(F, N * ndim, st.s);
-        rcp::hip_check(hipMemsetAsync(dcnt.p, 0, sizeof(unsigned int), st.s), "memset");
         rcp::StepArgs a;
         a.g = g;
         a.mag_out = dmag.p;
         a.partials = dpart.p;
-        a.counter = dcnt.p;
         a.result = dres.p;
         if (rcp::launch_mean_force(a, dF.p, 1, st.s)) throw rcp::HipFailure("mean force launch");
         double r[rcp::S_NUM];

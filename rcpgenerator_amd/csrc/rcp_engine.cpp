@@ -167,15 +167,14 @@ DeviceState::DeviceState(int N_, int nd_, bool with_fixed, bool exact_, hipStrea
         sort.vals = alloc<uint32_t>(N);
         sort.vals_alt = alloc<uint32_t>(N);
         sort.hist = alloc<uint32_t>(sort_hist_entries(N));
-        partials = alloc<double>((step_partial_blocks(N) + 1) * S_NUM);
+        partial_blocks = step_partial_blocks(N) + 1;
+        partials = alloc<double>(partial_blocks * S_NUM);
         result = alloc<double>(S_NUM);
-        counter = alloc<unsigned int>(1);
         if (exact) {
             mag = alloc<double>(N);
             ref_count = alloc<uint32_t>((size_t)N + 1);
             ref_start = alloc<uint32_t>((size_t)N + 1);
         }
-        hip_check(hipMemsetAsync(counter, 0, sizeof(unsigned int), s), "zero counter");
         hip_check(hipHostMalloc((void**)&h_result, S_NUM * sizeof(double),
                                 hipHostMallocDefault), "hipHostMalloc");
         ensure_scan(std::max(sort_hist_entries(N), (size_t)N) + 8);
@@ -209,6 +208,7 @@ void DeviceState::upload(const double* xh, const double* D0h, const int8_t* fixe
     if (launch_iota_u32(tag, N, s) || launch_iota_u32(inv_tag, N, s) ||
         launch_iota_u64(origin, N, s))
         throw HipFailure("iota launch");
+    inv_tag_valid = true;
     sync();
 }
 
@@ -243,7 +243,6 @@ void DeviceState::rebin(const Geometry& g, double reach)
     b.tag_o = tag_alt;
     b.origin_o = origin_alt;
     b.fixed_o = fixed_alt;
-    b.inv_tag = inv_tag;
     b.cell_of = cell_of;
     b.slot = slot;
     b.cell_count = cell_count;
@@ -258,6 +257,22 @@ void DeviceState::rebin(const Geometry& g, double reach)
     std::swap(origin, origin_alt);
     std::swap(fixed, fixed_alt);
     binned = true;
+    inv_tag_valid = false;
+}
+
+void DeviceState::ensure_inv_tag()
+{
+    if (inv_tag_valid) return;
+    if (launch_inv_tag(tag, inv_tag, N, s)) throw HipFailure("inv_tag launch");
+    inv_tag_valid = true;
+}
+
+void DeviceState::ensure_partials(size_t blocks)
+{
+    if (blocks <= partial_blocks) return;
+    release(partials);
+    partial_blocks = blocks + blocks / 4 + 1;
+    partials = alloc<double>(partial_blocks * S_NUM);
 }
 
 void DeviceState::build_rows(const Geometry& g, double kappa, double t_shell, int half,
@@ -305,8 +320,10 @@ void DeviceState::build_rows(const Geometry& g, double kappa, double t_shell, in
 
 void DeviceState::reference_reorder(const Geometry& g)
 {
+    ensure_inv_tag();
     if (launch_reference_reorder(g, x, tag, inv_tag, tag_tmp, sort, s))
         throw HipFailure("reference reorder launch");
+    inv_tag_valid = true;
 }
 
 void DeviceState::to_ref(const double* src_w, int width, std::vector<double>& out)
@@ -365,9 +382,12 @@ Engine::Engine(const rcp_run_config& c)
     if (c.ndim < 1 || c.ndim > (size_t)kMaxDim) throw ArgError("rcp: Ndim must be in 1..8");
     if (!c.positions || !c.diameters) throw ArgError("rcp: positions/diameters missing");
     if (c.N > 0x7fffffffull) throw ArgError("rcp: N too large");
-    if (c.neighbor_mode < 0 || c.neighbor_mode > 2) throw ArgError("rcp: neighbor_mode must be 0, 1 or 2");
-    if (c.neighbor_mode == 2 && c.exact_reductions)
+    if (c.neighbor_mode < 0 || c.neighbor_mode > 3)
+        throw ArgError("rcp: neighbor_mode must be 0, 1, 2 or 3");
+    if (c.neighbor_mode >= 2 && c.exact_reductions)
         throw ArgError("rcp: exact_reductions needs neighbour lists (neighbor_mode 0 or 1)");
+    if (c.neighbor_mode == 3 && c.ndim != 2 && c.ndim != 3)
+        throw ArgError("rcp: tiled steps (neighbor_mode 3) need ndim 2 or 3");
     N_ = c.N;
     nd_ = c.ndim;
     opt_ = c;
@@ -492,7 +512,7 @@ Engine::Engine(const rcp_run_config& c)
     kappa_D_ = 1.0;
     kappa_prev_ = 1.0;
     build_lists();
-    const Scalars s0 = step_device(false, 0.0, 1.0, 1.0, 0, nullptr, false);
+    const Scalars s0 = step_device(false, 0.0, 1.0, 1.0, 0, nullptr, Mode::List);
     finalize_scalars(s0, mu_);
 }
 
@@ -569,12 +589,40 @@ void Engine::build_lists()
     list_entries_ += st_->nnz;
 }
 
+bool Engine::tiles_supported() const
+{
+    if (opt_.exact_reductions || (nd_ != 2 && nd_ != 3)) return false;
+    return opt_.neighbor_mode == 0 || opt_.neighbor_mode == 3;
+}
+
+void Engine::tile_rebin(double skin)
+{
+    // Cells at least as wide as the largest contact distance plus the skin;
+    // the binning stays valid while every particle has moved less than half
+    // the skin (diameter growth and box stretch included).
+    begin_timer();
+    st_->rebin(geo_, dmax_now() + skin);
+    if (!plan_tiles((int)nd_, st_->grid, (long long)N_, has_fixed_ ? 1 : 0, tile_plan_))
+        throw RtError("rcp: no tile plan for this grid");
+    if (skin > 0.0)
+        hip_check(hipMemcpyAsync(st_->anchor, st_->x, N_ * nd_ * sizeof(double),
+                                 hipMemcpyDeviceToDevice, stream_), "anchor copy");
+    end_timer(t_build_ms_, t_build_n_);
+    tile_skin_ = skin;
+    tile_binned_ = true;
+    kappa_b_ = kappa_D_;
+    fh_drift_ = 0.0;
+    need_build_ = true;   // list rows (if any) are gone with the re-binning
+    ++tile_rebins_;
+}
+
 // --------------------------------------------------------------- device step
 
 Engine::Scalars Engine::step_device(bool do_adam, double alpha, double inv_b1, double inv_b2,
-                                    int use_rk, const double* rk, bool direct)
+                                    int use_rk, const double* rk, Mode mode)
 {
     DeviceState& S = *st_;
+    const bool direct = mode == Mode::Direct;
     StepArgs a;
     a.g = geo_;
     a.x_in = S.x;
@@ -584,7 +632,7 @@ Engine::Scalars Engine::step_device(bool do_adam, double alpha, double inv_b1, d
     a.D0 = S.D0;
     a.fixed = S.fixed;
     a.tag = S.tag;
-    a.x_anchor = direct ? nullptr : S.anchor;
+    a.x_anchor = direct || (mode == Mode::Tile && tile_skin_ <= 0.0) ? nullptr : S.anchor;
     a.row_start = S.row_start;
     a.nbr = S.nbr;
     a.grid = S.grid;
@@ -598,10 +646,14 @@ Engine::Scalars Engine::step_device(bool do_adam, double alpha, double inv_b1, d
     a.do_adam = do_adam ? 1 : 0;
     a.inv_tag = S.inv_tag;
     a.partials = S.partials;
-    a.counter = S.counter;
     a.result = S.result;
+    if (mode == Mode::Tile) S.ensure_partials((size_t)tile_plan_.blocks + 1);
+    if (opt_.exact_reductions) S.ensure_inv_tag();
+    a.partials = S.partials;
     begin_timer();
-    if (direct) {
+    if (mode == Mode::Tile) {
+        if (launch_step_tile(a, tile_plan_, stream_)) throw HipFailure("tile step launch");
+    } else if (direct) {
         if (launch_step_direct(a, stream_)) throw HipFailure("direct step launch");
     } else if (opt_.exact_reductions) {
         a.mag_out = S.mag;
@@ -798,6 +850,7 @@ void Engine::lock_target_phi(double box_volume)
     const double coeff = std::pow(M_PI, nd_ / 2.0) / std::tgamma(nd_ / 2.0 + 1.0);
     phi_ = coeff * S_D0_ / box_volume;
     need_build_ = true;
+    tile_invalid_ = true;
 }
 
 void Engine::release_cascade()
@@ -817,6 +870,7 @@ void Engine::release_cascade()
     shell0_ = reference_shell(D0);
     cascade_released_ = true;
     need_build_ = true;
+    tile_invalid_ = true;
 }
 
 bool Engine::step_once()
@@ -849,7 +903,8 @@ bool Engine::step_once()
         geo_.inv_box[nd_ - 1] = 1.0 / box_[nd_ - 1];
         if (launch_scale_last_dim(st_->x, (int)N_, (int)nd_, kappa_, stream_))
             throw HipFailure("scale launch");
-        fh_drift_ += std::fabs(box_[nd_ - 1] - L_old);
+        fh_step_ = std::fabs(box_[nd_ - 1] - L_old);
+        fh_drift_ += fh_step_;
     }
     phi_ = current_volume / box_volume;
     if (opt_.has_target_phi && !target_locked_ && !runtime_fix_) {
@@ -878,7 +933,20 @@ bool Engine::step_once()
     // list live long enough to pay for its build.
     const double growth_step = D0max_free_ * std::fabs(kappa_D_ - kappa_prev_);
     kappa_prev_ = kappa_D_;
-    if (opt_.neighbor_mode == 2) {
+    if (tiles_supported()) {
+        mode_ = Mode::Tile;
+        // the binning holds while no particle can have crossed half the skin
+        const double growth = std::max(0.0, D0max_free_ * (kappa_D_ - kappa_b_));
+        const double reach = 2.0 * std::sqrt(maxdisp2_) + growth + 2.0 * fh_drift_;
+        const bool stale = tile_skin_ <= 0.0 || !(reach < tile_skin_ * (1.0 - 1e-6));
+        if (!tile_binned_ || stale || tile_invalid_) {
+            // a skin pays off once a binning lives two steps or more
+            const double per_step = 2.0 * std::sqrt(maxstep2_) + growth_step + 2.0 * fh_step_;
+            const double shell = shell0_ * kappa_D_;
+            tile_rebin(per_step * 2.0 < shell ? shell : 0.0);
+            tile_invalid_ = false;
+        }
+    } else if (opt_.neighbor_mode == 2) {
         mode_ = Mode::Direct;
     } else if (opt_.neighbor_mode == 0 && !opt_.exact_reductions && mode_ == Mode::Direct) {
         const double per_step = 2.0 * std::sqrt(maxstep2_) + growth_step;
@@ -903,7 +971,9 @@ bool Engine::step_once()
         }
     }
     const bool direct = (mode_ == Mode::Direct);
-    if (direct) {
+    if (mode_ == Mode::Tile) {
+        ++tile_steps_;
+    } else if (direct) {
         begin_timer();
         st_->rebin(geo_, dmax_now());
         end_timer(t_build_ms_, t_build_n_);
@@ -928,7 +998,7 @@ bool Engine::step_once()
             throw HipFailure("rk launch");
         rk = st_->rk;
     }
-    const Scalars s = step_device(true, alpha_, inv_b1, inv_b2, has_fixed_ ? 1 : 0, rk, direct);
+    const Scalars s = step_device(true, alpha_, inv_b1, inv_b2, has_fixed_ ? 1 : 0, rk, mode_);
     finalize_scalars(s, mu_eff);
     force_magnitude_ = mean_force_ / Fmean_;
     F_h_[step - 1] = force_magnitude_;

@@ -64,6 +64,10 @@ public:
     void build_rows(const Geometry& g, double kappa, double t_shell, int half, int sortwalk_rc);
     // reference reorder of the tags
     void reference_reorder(const Geometry& g);
+    // inv_tag for the current order (rebuilt after a re-binning on demand)
+    void ensure_inv_tag();
+    // partial-sum rows for a step launch of `blocks` workgroups
+    void ensure_partials(size_t blocks);
     void to_ref(const double* src_w, int width, std::vector<double>& out);
     void d0_ref(std::vector<double>& D0, std::vector<int8_t>& fixed);
     void set_d0_ref(const std::vector<double>& D0, const std::vector<int8_t>& fixed);
@@ -76,7 +80,8 @@ public:
     hipStream_t s;
     CellGrid grid;
     uint32_t nnz = 0;
-    bool binned = false;
+    bool binned = false, inv_tag_valid = true;
+    size_t partial_blocks = 0;
     // state (working order); *_alt are re-binning destinations
     double *x = nullptr, *x_alt = nullptr, *m = nullptr, *m_alt = nullptr, *v = nullptr,
            *v_alt = nullptr, *D0 = nullptr, *D0_alt = nullptr, *anchor = nullptr,
@@ -93,7 +98,6 @@ public:
     uint32_t *ref_count = nullptr, *ref_start = nullptr;
     // reductions
     double *partials = nullptr, *result = nullptr, *h_result = nullptr;
-    unsigned int* counter = nullptr;
     // sorting
     SortTmp sort;
     char* scan_tmp = nullptr;
@@ -147,6 +151,8 @@ public:
     uint64_t reorders() const { return reorders_; }
     uint64_t direct_steps() const { return direct_steps_total_; }
     uint64_t overflow_steps() const { return overflow_steps_; }
+    uint64_t tile_steps() const { return tile_steps_; }
+    uint64_t tile_rebins() const { return tile_rebins_; }
     void timing(double& step_ms, uint64_t& steps, double& build_ms, uint64_t& builds) const
     {
         step_ms = t_step_ms_;
@@ -158,13 +164,15 @@ public:
 private:
     struct Scalars { double v[S_NUM]; };
     enum class Osc { Inactive, Active, Final, Done };
-    enum class Mode { List, Direct };
+    enum class Mode { List, Direct, Tile };
 
     void recompute_d0_cache(const std::vector<double>& D0, const std::vector<int8_t>& fixed);
     double dmax_now() const;
     void build_lists();
     Scalars step_device(bool do_adam, double alpha, double inv_b1, double inv_b2, int use_rk,
-                        const double* rk, bool direct);
+                        const double* rk, Mode mode);
+    void tile_rebin(double skin);
+    bool tiles_supported() const;
     void finalize_scalars(const Scalars& s, double mu_eff);
     bool schedule_top(std::size_t step);
     void schedule_alpha(std::size_t step);
@@ -218,6 +226,10 @@ private:
     double t_list_ = 0.0, kappa_b_ = 1.0, fh_drift_ = 0.0, kappa_prev_ = 1.0;
     bool need_build_ = true;
     std::size_t list_age_ = 0, direct_run_ = 0;
+    double tile_skin_ = 0.0, fh_step_ = 0.0;
+    bool tile_binned_ = false, tile_invalid_ = false;
+    uint64_t tile_rebins_ = 0, tile_steps_ = 0;
+    TilePlan tile_plan_;
     uint64_t list_builds_ = 0, list_entries_ = 0, reorders_ = 0, direct_steps_total_ = 0,
              overflow_steps_ = 0;
 

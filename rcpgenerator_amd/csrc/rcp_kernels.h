@@ -78,10 +78,28 @@ struct StepArgs {
     double* mag_out = nullptr;       // optional N: sqrt(|F|^2) or -1 (exact mode)
     double* contrib = nullptr;       // optional 4 doubles per row slot (exact mode)
     const uint32_t* inv_tag = nullptr; // N: working position of reference index
-    double* partials = nullptr;      // gridDim * S_NUM
-    unsigned int* counter = nullptr; // last-block ticket
+    double* partials = nullptr;      // gridDim * S_NUM rows, folded into result
     double* result = nullptr;        // S_NUM totals
+    // tiled step (launch_step_tile): plan from plan_tiles
+    int tile_len = 0, tiles_per_row = 0, hc_cap = 0, h_cap = 0, list_cap = 0;
 };
+
+// Tiled step over the binned working order (ndim 2 and 3).  One workgroup
+// owns tile_len consecutive cells of one row along axis 0; it stages the
+// particles of those cells and of the neighbouring rows' cells (the halo)
+// in LDS with coalesced loads and scans each particle's cell stencil there.
+// A tile whose halo exceeds h_cap reads the same cells from global memory
+// instead (same arithmetic, same result).  Cells need only be at least as
+// wide as the contact reach at binning time plus twice the largest
+// displacement since, so one binning can serve several steps.
+struct TilePlan {
+    int block = 0, tile_len = 0, tiles_per_row = 0, hc_cap = 0, h_cap = 0, list_cap = 0;
+    long long tiles = 0, blocks = 0;   // tiles; persistent workgroups
+    size_t lds_bytes = 0;
+};
+// false when the tiled step does not handle this shape (ndim not 2 or 3)
+bool plan_tiles(int ndim, const CellGrid& grid, long long N, int use_rk, TilePlan& p);
+int launch_step_tile(const StepArgs& a, const TilePlan& p, hipStream_t s);
 
 // blocks of the widest step launch over n particles (partials: +1 rows)
 size_t step_partial_blocks(size_t n);
@@ -95,7 +113,8 @@ int launch_exact_step(const StepArgs& a, uint32_t* ref_count, uint32_t* ref_star
 
 // ---------------------------------------------------------------- binning
 // Re-binning to a new working order: old-order arrays are gathered into
-// cell order (within a cell by ascending tag), inv_tag is rebuilt.
+// cell order (within a cell by previous working position).  inv_tag is not
+// maintained here (launch_inv_tag rebuilds it when needed).
 struct BinArgs {
     Geometry g;
     CellGrid grid;
@@ -115,7 +134,6 @@ struct BinArgs {
     uint32_t* tag_o = nullptr;
     unsigned long long* origin_o = nullptr;
     int8_t* fixed_o = nullptr;
-    uint32_t* inv_tag = nullptr;
     // scratch
     uint32_t* cell_of = nullptr;       // N
     uint32_t* slot = nullptr;          // N
@@ -124,6 +142,8 @@ struct BinArgs {
     uint32_t* perm = nullptr;          // N: old position of new position q
 };
 int launch_rebin(BinArgs& a, void* scan_tmp, hipStream_t s);
+// inv_tag[tag[w]] = w
+int launch_inv_tag(const uint32_t* tag, uint32_t* inv_tag, int N, hipStream_t s);
 
 // ---------------------------------------------------------------- lists
 struct ListArgs {

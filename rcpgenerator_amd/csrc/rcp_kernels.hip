@@ -120,38 +120,41 @@ __device__ void block_reduce(double (&v)[S_NUM], double* out)
     __syncthreads();
 }
 
-// The last block to finish folds all block partials (in block order) into
-// a.result and re-arms the ticket.
+// Each workgroup stores its row of partial sums; k_fold combines the rows
+// in a fixed order in a separate launch (the launch boundary makes the rows
+// visible, so no workgroup needs a device-scope release, which on a
+// multi-XCD part would write back its whole L2).
 template <int BLOCK>
-__device__ void finish_grid(const StepArgs& a, double (&blk)[S_NUM])
+__device__ __forceinline__ void finish_grid(const StepArgs& a, double (&blk)[S_NUM])
 {
-    __shared__ int is_last;
     if (threadIdx.x == 0) {
+#pragma unroll
         for (int k = 0; k < S_NUM; ++k) a.partials[(size_t)blockIdx.x * S_NUM + k] = blk[k];
-        unsigned int ticket = __hip_atomic_fetch_add(a.counter, 1u, __ATOMIC_ACQ_REL,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (ticket == gridDim.x - 1);
     }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
+}
+
+constexpr int kFoldBlock = 1024;
+
+__global__ __launch_bounds__(kFoldBlock) void k_fold(const double* partials, unsigned rows,
+                                                     double* result)
+{
     double v[S_NUM];
 #pragma unroll
     for (int k = 0; k < S_NUM; ++k) v[k] = 0.0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
-        const double* p = a.partials + (size_t)b * S_NUM;
+    for (unsigned b = threadIdx.x; b < rows; b += kFoldBlock) {
+        const double* p = partials + (size_t)b * S_NUM;
 #pragma unroll
         for (int k = 0; k < S_NUM; ++k) {
-            double t = __builtin_nontemporal_load(p + k);
+            const double t = p[k];
             if (is_max_slot(k)) v[k] = (t > v[k]) ? t : v[k];
             else v[k] += t;
         }
     }
     double tot[S_NUM];
-    block_reduce<BLOCK>(v, tot);
+    block_reduce<kFoldBlock>(v, tot);
     if (threadIdx.x == 0) {
-        for (int k = 0; k < S_NUM; ++k) a.result[k] = tot[k];
-        __hip_atomic_store(a.counter, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+        for (int k = 0; k < S_NUM; ++k) result[k] = tot[k];
     }
 }
 
@@ -177,18 +180,17 @@ __device__ __forceinline__ void load_own(const StepArgs& a, int q, Own<ND>& o)
     o.tag = a.tag[q];
 }
 
-// Contact of particle o with partner qj (RG:4086-4139): adds the force to F
-// and, when the partner's tag is larger, the pair terms to acc.  Returns
-// whether the pair terms were counted (exact mode records them).
+// Contact of particle o with a partner at xj of diameter Dj, cascade weight
+// rkj and reference index tj (RG:4086-4139): adds the force to F and, when
+// the partner's tag is larger, the pair terms to acc.  Returns whether the
+// pair terms were counted (exact mode records them).  Every step kernel goes
+// through this one function, so all paths round identically.
 template <int ND>
-__device__ __forceinline__ bool contact(const StepArgs& a, const Own<ND>& o, uint32_t qj,
-                                        uint32_t tj, double (&F)[ND], double (&acc)[S_NUM],
-                                        double* rec)
+__device__ __forceinline__ bool pair_contact(const StepArgs& a, const Own<ND>& o,
+                                             const double (&xj)[ND], double Dj, double rkj,
+                                             uint32_t tj, double (&F)[ND],
+                                             double (&acc)[S_NUM], double* rec)
 {
-    const double* xj = a.x_in + (size_t)qj * ND;
-    const bool fj = a.fixed && a.fixed[qj];
-    const double D0j = a.D0[qj];
-    const double Dj = fj ? D0j : D0j * a.kappa;
     const double r = 0.5 * (o.D + Dj);
     double dl[ND];
     double d2 = 0.0;
@@ -210,7 +212,6 @@ __device__ __forceinline__ bool contact(const StepArgs& a, const Own<ND>& o, uin
     acc[S_CNT] += 1.0;
     acc[S_LC] += r;
     acc[S_MAXOV] = (ov > acc[S_MAXOV]) ? ov : acc[S_MAXOV];
-    const double rkj = a.rk ? a.rk[qj] : (fj ? 0.0 : Dj * 0.5);
     const double dkt = (a.use_rk ? (o.rk + rkj) : r) * (dist - r);
     acc[S_DK] = acc[S_DK] + dkt;
     acc[S_FM] = acc[S_FM] + fm;
@@ -221,6 +222,45 @@ __device__ __forceinline__ bool contact(const StepArgs& a, const Own<ND>& o, uin
         rec[3] = fm;
     }
     return true;
+}
+
+// Partner data of working position qj as the step kernels see it.
+template <int ND>
+__device__ __forceinline__ void load_partner(const StepArgs& a, uint32_t qj, double (&xj)[ND],
+                                             double& Dj, double& rkj)
+{
+#pragma unroll
+    for (int d = 0; d < ND; ++d) xj[d] = a.x_in[(size_t)qj * ND + d];
+    const bool fj = a.fixed && a.fixed[qj];
+    const double D0j = a.D0[qj];
+    Dj = fj ? D0j : D0j * a.kappa;
+    rkj = a.use_rk ? (a.rk ? a.rk[qj] : (fj ? 0.0 : Dj * 0.5)) : 0.0;
+}
+
+template <int ND>
+__device__ __forceinline__ bool contact(const StepArgs& a, const Own<ND>& o, uint32_t qj,
+                                        uint32_t tj, double (&F)[ND], double (&acc)[S_NUM],
+                                        double* rec)
+{
+    double xj[ND], Dj, rkj;
+    load_partner<ND>(a, qj, xj, Dj, rkj);
+    return pair_contact<ND>(a, o, xj, Dj, rkj, tj, F, acc, rec);
+}
+
+// Overlap test of the pair arithmetic (d2 < r^2), without side effects.
+template <int ND>
+__device__ __forceinline__ bool overlap_at(const StepArgs& a, const Own<ND>& o,
+                                           const double (&xj)[ND], double Dj)
+{
+    const double r = 0.5 * (o.D + Dj);
+    double d2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        double delta = xj[d] - o.x[d];
+        delta = mic(delta, a.g.inv_box[d], a.g.box[d]);
+        d2 += delta * delta;
+    }
+    return d2 < r * r;
 }
 
 // Walls, curved container, mean-force term, ADAM + wrap and displacements
@@ -290,7 +330,7 @@ __device__ __forceinline__ void particle_tail(const StepArgs& a, int q, const Ow
         acc[S_MCNT] += 1.0;
     }
     if (a.mag_out) a.mag_out[q] = mag;
-    acc[S_SUMD] = (a.use_rk && !a.rk && o.fixed) ? 0.0 : o.D;
+    acc[S_SUMD] += (a.use_rk && !a.rk && o.fixed) ? 0.0 : o.D;
     if (a.F_out) {
 #pragma unroll
         for (int d = 0; d < ND; ++d) a.F_out[(size_t)q * ND + d] = F[d];
@@ -315,8 +355,8 @@ __device__ __forceinline__ void particle_tail(const StepArgs& a, int q, const Ow
                 d2a += da * da;
             }
         }
-        acc[S_MAXDISP2] = d2a;
-        acc[S_MAXSTEP2] = d2s;
+        acc[S_MAXDISP2] = (d2a > acc[S_MAXDISP2]) ? d2a : acc[S_MAXDISP2];
+        acc[S_MAXSTEP2] = (d2s > acc[S_MAXSTEP2]) ? d2s : acc[S_MAXSTEP2];
     }
 }
 
@@ -400,19 +440,9 @@ __device__ __forceinline__ void for_each_candidate(const StepArgs& a, const doub
 template <int ND>
 __device__ __forceinline__ bool overlaps(const StepArgs& a, const Own<ND>& o, uint32_t qj)
 {
-    const double* xj = a.x_in + (size_t)qj * ND;
-    const bool fj = a.fixed && a.fixed[qj];
-    const double D0j = a.D0[qj];
-    const double Dj = fj ? D0j : D0j * a.kappa;
-    const double r = 0.5 * (o.D + Dj);
-    double d2 = 0.0;
-#pragma unroll
-    for (int d = 0; d < ND; ++d) {
-        double delta = xj[d] - o.x[d];
-        delta = mic(delta, a.g.inv_box[d], a.g.box[d]);
-        d2 += delta * delta;
-    }
-    return d2 < r * r;
+    double xj[ND], Dj, rkj;
+    load_partner<ND>(a, qj, xj, Dj, rkj);
+    return overlap_at<ND>(a, o, xj, Dj);
 }
 
 template <int ND>
@@ -478,6 +508,352 @@ __global__ __launch_bounds__(kDirectBlock) void k_step_direct(StepArgs a)
     double blk[S_NUM];
     block_reduce<kDirectBlock>(acc, blk);
     finish_grid<kDirectBlock>(a, blk);
+}
+
+// ------------------------------------------------------------ tiled step
+
+// LDS layout of one tile (offsets in bytes), shared by host and device.
+struct TileLayout {
+    int hc_first, hc_off, sx, sD, srk, stag, lists, bytes;
+};
+
+__host__ __device__ inline TileLayout tile_layout(int nd, int hc_cap, int h_cap, int list_cap,
+                                                  int block, bool rk)
+{
+    TileLayout L;
+    int o = 0;
+    L.hc_first = o;
+    o += hc_cap * 4;
+    L.hc_off = o;
+    o += (hc_cap + 1) * 4;
+    o = (o + 15) & ~15;
+    L.sx = o;
+    o += nd * h_cap * 8;
+    L.sD = o;
+    o += h_cap * 8;
+    L.srk = o;
+    o += rk ? h_cap * 8 : 0;
+    L.stag = o;
+    o += h_cap * 4;
+    o = (o + 15) & ~15;
+    L.lists = o;
+    o += list_cap * block * 2;
+    L.bytes = o;
+    return L;
+}
+
+// Exclusive scan in place of v[0..n) (n <= hc_cap) by one workgroup;
+// v[n] = total.  Each thread scans a contiguous chunk.
+template <int BLOCK>
+__device__ void block_scan_inplace(uint32_t* v, int n)
+{
+    __shared__ uint32_t wtot[BLOCK / kWave];
+    const int tid = threadIdx.x, lane = tid % kWave, w = tid / kWave;
+    const int per = (n + BLOCK - 1) / BLOCK;
+    const int b0 = tid * per;
+    const int b1 = (b0 + per < n) ? b0 + per : n;
+    uint32_t sum = 0;
+    for (int k = b0; k < b1; ++k) sum += v[k];
+    uint32_t incl = sum;
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o, kWave);
+        if (lane >= o) incl += y;
+    }
+    if (lane == kWave - 1) wtot[w] = incl;
+    __syncthreads();
+    uint32_t run = incl - sum;
+    for (int q = 0; q < w; ++q) run += wtot[q];
+    for (int k = b0; k < b1; ++k) {
+        const uint32_t c = v[k];
+        v[k] = run;
+        run += c;
+    }
+    if (tid == BLOCK - 1) {
+        uint32_t tot = 0;
+        for (int q = 0; q < BLOCK / kWave; ++q) tot += wtot[q];
+        v[n] = tot;
+    }
+    __syncthreads();
+}
+
+// largest h in [lo, hi) with off[h] <= i (off ascending)
+__device__ __forceinline__ int find_cell(const uint32_t* off, int lo, int hi, uint32_t i)
+{
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Staged (LDS) or global view of a tile's particles: slot j is an LDS
+// index when LDS is true, a working position otherwise.
+template <int ND, bool LDS>
+struct TileSrc {
+    const StepArgs* a;
+    const double* sx;
+    const double* sD;
+    const double* srk;
+    const uint32_t* stag;
+    int h_cap;
+    bool rk;
+    __device__ __forceinline__ void get(uint32_t j, double (&xj)[ND], double& Dj,
+                                        double& rkj) const
+    {
+        if (LDS) {
+#pragma unroll
+            for (int d = 0; d < ND; ++d) xj[d] = sx[d * h_cap + j];
+            Dj = sD[j];
+            rkj = rk ? srk[j] : 0.0;
+        } else {
+            load_partner<ND>(*a, j, xj, Dj, rkj);
+        }
+    }
+    __device__ __forceinline__ void pos(uint32_t j, double (&xj)[ND], double& Dj) const
+    {
+        if (LDS) {
+#pragma unroll
+            for (int d = 0; d < ND; ++d) xj[d] = sx[d * h_cap + j];
+            Dj = sD[j];
+        } else {
+            double rkj;
+            load_partner<ND>(*a, j, xj, Dj, rkj);
+        }
+    }
+    __device__ __forceinline__ uint32_t tag(uint32_t j) const { return LDS ? stag[j] : a->tag[j]; }
+};
+
+// Forces, pair terms and the ADAM tail of the tile's own particles.
+template <int ND, int BLOCK, bool LDS>
+__device__ __forceinline__ void tile_particles(const StepArgs& a, const TileSrc<ND, LDS>& src,
+                                               const uint32_t* hc_first, const uint32_t* hc_off,
+                                               uint16_t* mylist, int nrows, int kx, int span_x,
+                                               int own_h0, int ncx, uint32_t own_lo,
+                                               uint32_t own_hi, uint32_t q_own,
+                                               double (&acc)[S_NUM])
+{
+    const int cap = a.list_cap;
+    for (uint32_t i = own_lo + threadIdx.x; i < own_hi; i += BLOCK) {
+        const uint32_t q = q_own + (i - own_lo);
+        Own<ND> o;
+        load_own<ND>(a, (int)q, o);
+        const int kc = find_cell(hc_off, own_h0, own_h0 + ncx, i) - (own_h0 - span_x);
+        const uint32_t self = LDS ? i : q;
+        double F[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) F[d] = 0.0;
+        int n = 0;
+        auto push = [&](uint32_t j) {
+            if (n < cap) mylist[n * BLOCK] = (uint16_t)j;
+            ++n;
+        };
+        if (LDS) {
+            // the x-neighbour cells of a row are adjacent in LDS: one range
+            // per row, four candidates per pass so their reads overlap
+            for (int ro = 0; ro < nrows; ++ro) {
+                const int hb = ro * kx + kc;
+                const uint32_t j1 = hc_off[hb + span_x + 1];
+                uint32_t j = hc_off[hb - span_x];
+                for (; j + 4 <= j1; j += 4) {
+                    double xj[4][ND], Dj[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) src.pos(j + u, xj[u], Dj[u]);
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (j + u != self && overlap_at<ND>(a, o, xj[u], Dj[u])) push(j + u);
+                }
+                for (; j < j1; ++j) {
+                    double xj[ND], Dj;
+                    src.pos(j, xj, Dj);
+                    if (j != self && overlap_at<ND>(a, o, xj, Dj)) push(j);
+                }
+            }
+        } else {
+            for (int ro = 0; ro < nrows; ++ro) {
+                for (int dk = -span_x; dk <= span_x; ++dk) {
+                    const int h = ro * kx + kc + dk;
+                    const uint32_t j0 = hc_first[h], cnt = hc_off[h + 1] - hc_off[h];
+                    for (uint32_t j = j0; j < j0 + cnt; ++j) {
+                        double xj[ND], Dj;
+                        src.pos(j, xj, Dj);
+                        if (j != self && overlap_at<ND>(a, o, xj, Dj)) push(j);
+                    }
+                }
+            }
+        }
+        if (n <= cap && (LDS || n == 0)) {
+            for (int p = 1; p < n; ++p) {
+                const uint16_t jk = mylist[p * BLOCK];
+                const uint32_t tk = src.tag(jk);
+                int k = p;
+                while (k > 0 && src.tag(mylist[(k - 1) * BLOCK]) > tk) {
+                    mylist[k * BLOCK] = mylist[(k - 1) * BLOCK];
+                    --k;
+                }
+                mylist[k * BLOCK] = jk;
+            }
+            for (int p = 0; p < n; ++p) {
+                const uint32_t j = mylist[p * BLOCK];
+                double xj[ND], Dj, rkj;
+                src.get(j, xj, Dj, rkj);
+                pair_contact<ND>(a, o, xj, Dj, rkj, src.tag(j), F, acc, nullptr);
+            }
+        } else {
+            // more contacts than the buffer (or positions beyond 16 bits in
+            // the global view): take them in tag order by repeated scans
+            if (n > cap) acc[S_OVERFLOW] += 1.0;
+            long long last = -1;
+            while (true) {
+                unsigned long long best = ~0ull;
+                for (int ro = 0; ro < nrows; ++ro) {
+                    for (int dk = -span_x; dk <= span_x; ++dk) {
+                        const int h = ro * kx + kc + dk;
+                        const uint32_t j0 = LDS ? hc_off[h] : hc_first[h];
+                        const uint32_t cnt = hc_off[h + 1] - hc_off[h];
+                        for (uint32_t j = j0; j < j0 + cnt; ++j) {
+                            if (j == self) continue;
+                            const uint32_t tj = src.tag(j);
+                            if ((long long)tj <= last) continue;
+                            const unsigned long long key = ((unsigned long long)tj << 32) | j;
+                            if (key >= best) continue;
+                            double xj[ND], Dj;
+                            src.pos(j, xj, Dj);
+                            if (overlap_at<ND>(a, o, xj, Dj)) best = key;
+                        }
+                    }
+                }
+                if (best == ~0ull) break;
+                const uint32_t j = (uint32_t)(best & 0xffffffffull);
+                double xj[ND], Dj, rkj;
+                src.get(j, xj, Dj, rkj);
+                pair_contact<ND>(a, o, xj, Dj, rkj, (uint32_t)(best >> 32), F, acc, nullptr);
+                last = (long long)(best >> 32);
+            }
+        }
+        particle_tail<ND>(a, (int)q, o, F, acc);
+    }
+}
+
+constexpr int kXcds = 8;   // MI355X: 8 accelerator complex dies, blocks dealt round-robin
+
+template <int ND, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_step_tile(StepArgs a, long long ntiles)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const bool rk_lds = a.use_rk != 0;
+    const TileLayout L = tile_layout(ND, a.hc_cap, a.h_cap, a.list_cap, BLOCK, rk_lds);
+    uint32_t* hc_first = reinterpret_cast<uint32_t*>(smem + L.hc_first);
+    uint32_t* hc_off = reinterpret_cast<uint32_t*>(smem + L.hc_off);
+    double* sx = reinterpret_cast<double*>(smem + L.sx);
+    double* sD = reinterpret_cast<double*>(smem + L.sD);
+    double* srk = reinterpret_cast<double*>(smem + L.srk);
+    uint32_t* stag = reinterpret_cast<uint32_t*>(smem + L.stag);
+    uint16_t* lists = reinterpret_cast<uint16_t*>(smem + L.lists);
+    const int tid = threadIdx.x;
+    const int h_cap = a.h_cap;
+    double acc[S_NUM];
+#pragma unroll
+    for (int k = 0; k < S_NUM; ++k) acc[k] = 0.0;
+
+    // Workgroups are dealt to the XCDs round-robin; the workgroups of one XCD
+    // take one contiguous band of tiles, so neighbouring tiles (which share
+    // halo cells) hit the same L2.
+    const unsigned G = gridDim.x;
+    const unsigned xcds = G >= (unsigned)kXcds ? kXcds : 1u;
+    const unsigned xcd = blockIdx.x % xcds, lane_blk = blockIdx.x / xcds;
+    const unsigned per_xcd = (G - xcd + xcds - 1) / xcds;   // workgroups on this XCD
+    const long long band = (ntiles + xcds - 1) / xcds;
+    const long long t_begin = (long long)xcd * band;
+    const long long t_end = (t_begin + band < ntiles) ? t_begin + band : ntiles;
+    for (long long tix = t_begin + lane_blk; tix < t_end; tix += per_xcd) {
+    __syncthreads();   // LDS of the previous tile is free
+
+    // ---- the tile and its halo cells
+    const int nx = a.grid.nc[0];
+    const int span_x = nx >= 3 ? 1 : 0;
+    const int tile = (int)(tix % a.tiles_per_row);
+    const int row = (int)(tix / a.tiles_per_row);
+    const int cx_lo = tile * a.tile_len;
+    const int cx_hi = (cx_lo + a.tile_len < nx) ? cx_lo + a.tile_len : nx;
+    const int ncx = cx_hi - cx_lo;
+    const int kx = ncx + 2 * span_x;
+    int rc[ND], rspan[ND];
+    int nrows = 1, ro_c = 0;
+    {
+        int r = row;
+#pragma unroll
+        for (int d = 1; d < ND; ++d) {
+            rc[d] = r % a.grid.nc[d];
+            r /= a.grid.nc[d];
+            rspan[d] = a.grid.nc[d] >= 3 ? 3 : 1;
+            if (rspan[d] == 3) ro_c += nrows;
+            nrows *= rspan[d];
+        }
+    }
+    const int HC = nrows * kx;
+    for (int h = tid; h < HC; h += BLOCK) {
+        const int ro = h / kx, k = h - ro * kx;
+        int cx = cx_lo - span_x + k;
+        if (cx < 0) cx += nx;
+        if (cx >= nx) cx -= nx;
+        long long cc = cx, stride = nx;
+        int t = ro;
+#pragma unroll
+        for (int d = 1; d < ND; ++d) {
+            int cd = rc[d];
+            if (rspan[d] == 3) {
+                cd += t % 3 - 1;
+                t /= 3;
+                if (cd < 0) cd += a.grid.nc[d];
+                if (cd >= a.grid.nc[d]) cd -= a.grid.nc[d];
+            }
+            cc += (long long)cd * stride;
+            stride *= a.grid.nc[d];
+        }
+        const uint32_t s0 = a.cell_start[cc];
+        hc_first[h] = s0;
+        hc_off[h] = a.cell_start[cc + 1] - s0;
+    }
+    __syncthreads();
+    block_scan_inplace<BLOCK>(hc_off, HC);
+    const uint32_t H = hc_off[HC];
+    const bool lds = H <= (uint32_t)h_cap;
+    const int own_h0 = ro_c * kx + span_x;
+    const uint32_t own_lo = hc_off[own_h0], own_hi = hc_off[own_h0 + ncx];
+    const uint32_t q_own = hc_first[own_h0];
+
+    // ---- stage the halo (coalesced: cells of a row are contiguous)
+    if (lds && own_hi > own_lo) {
+        for (uint32_t i = tid; i < H; i += BLOCK) {
+            const int h = find_cell(hc_off, 0, HC, i);
+            const uint32_t p = hc_first[h] + (i - hc_off[h]);
+#pragma unroll
+            for (int d = 0; d < ND; ++d) sx[d * h_cap + i] = a.x_in[(size_t)p * ND + d];
+            const bool fp = a.fixed && a.fixed[p];
+            const double D0p = a.D0[p];
+            const double Dp = fp ? D0p : D0p * a.kappa;
+            sD[i] = Dp;
+            if (rk_lds) srk[i] = a.rk ? a.rk[p] : (fp ? 0.0 : Dp * 0.5);
+            stag[i] = a.tag[p];
+        }
+    }
+    __syncthreads();
+
+    uint16_t* mylist = lists + tid;
+    if (lds) {
+        const TileSrc<ND, true> src{&a, sx, sD, srk, stag, h_cap, rk_lds};
+        tile_particles<ND, BLOCK, true>(a, src, hc_first, hc_off, mylist, nrows, kx, span_x,
+                                        own_h0, ncx, own_lo, own_hi, q_own, acc);
+    } else {
+        const TileSrc<ND, false> src{&a, sx, sD, srk, stag, h_cap, rk_lds};
+        tile_particles<ND, BLOCK, false>(a, src, hc_first, hc_off, mylist, nrows, kx, span_x,
+                                         own_h0, ncx, own_lo, own_hi, q_own, acc);
+    }
+    }   // tiles
+    double blk[S_NUM];
+    block_reduce<BLOCK>(acc, blk);
+    finish_grid<BLOCK>(a, blk);
 }
 
 // -------------------------------------------------------- exact reduction
@@ -665,7 +1041,8 @@ __global__ __launch_bounds__(kBlock) void k_place(BinArgs a)
     a.perm[a.cell_start[a.cell_of[w]] + a.slot[w]] = (uint32_t)w;
 }
 
-// One thread per cell: order the cell's members by tag (deterministic).
+// One thread per cell: order the cell's members by their previous working
+// position, which makes the new order independent of atomic arrival order.
 __global__ __launch_bounds__(kBlock) void k_cell_order(BinArgs a)
 {
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -673,10 +1050,10 @@ __global__ __launch_bounds__(kBlock) void k_cell_order(BinArgs a)
     const uint32_t q0 = a.cell_start[c], q1 = a.cell_start[c + 1];
     for (uint32_t p = q0 + 1; p < q1; ++p) {
         const uint32_t w = a.perm[p];
-        const uint32_t t = a.tag[w];
         uint32_t k = p;
-        while (k > q0 && a.tag[a.perm[k - 1]] > t) {
-            a.perm[k] = a.perm[k - 1];
+        uint32_t prev;
+        while (k > q0 && (prev = a.perm[k - 1]) > w) {
+            a.perm[k] = prev;
             --k;
         }
         a.perm[k] = w;
@@ -696,9 +1073,7 @@ __global__ __launch_bounds__(kBlock) void k_permute(BinArgs a)
         a.v_o[(size_t)q * ND + d] = a.v[(size_t)w * ND + d];
     }
     a.D0_o[q] = a.D0[w];
-    const uint32_t t = a.tag[w];
-    a.tag_o[q] = t;
-    a.inv_tag[t] = (uint32_t)q;
+    a.tag_o[q] = a.tag[w];
     a.origin_o[q] = a.origin[w];
     if (a.fixed) a.fixed_o[q] = a.fixed[w];
 }
@@ -907,6 +1282,12 @@ __global__ __launch_bounds__(kBlock) void k_tag_rank(const uint32_t* sorted_old,
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r < N) rank[sorted_old[r]] = (uint32_t)r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_inv_tag(const uint32_t* tag, uint32_t* inv_tag, int N)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < N) inv_tag[tag[w]] = (uint32_t)w;
 }
 
 __global__ __launch_bounds__(kBlock) void k_tag_apply(uint32_t* tag, uint32_t* inv_tag,
@@ -1156,11 +1537,98 @@ size_t step_partial_blocks(size_t n)
     return (n + kDirectBlock - 1) / kDirectBlock;
 }
 
+constexpr int kTileBlock = 128;
+
+// Resident workgroups of the tiled kernel on the current device for a given
+// LDS size (all of them run at once, so the tiles split evenly).
+long long tile_grid(int ndim, size_t lds_bytes)
+{
+    // the answer depends on the device and the LDS size only: keep the last
+    thread_local int c_dev = -1, c_nd = 0;
+    thread_local size_t c_lds = 0;
+    thread_local long long c_val = 0;
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) == hipSuccess && dev == c_dev && ndim == c_nd && lds_bytes == c_lds)
+        return c_val;
+    if (hipGetDevice(&dev) != hipSuccess) return 1024;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 1024;
+    hipError_t e = ndim == 2
+        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_step_tile<2, kTileBlock>,
+                                                       kTileBlock, lds_bytes)
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_step_tile<3, kTileBlock>,
+                                                       kTileBlock, lds_bytes);
+    if (e != hipSuccess || per_cu < 1) per_cu = 1;
+    c_dev = dev;
+    c_nd = ndim;
+    c_lds = lds_bytes;
+    c_val = (long long)cus * per_cu;
+    return c_val;
+}
+
+bool plan_tiles(int ndim, const CellGrid& grid, long long N, int use_rk, TilePlan& p)
+{
+    if (ndim != 2 && ndim != 3) return false;
+    p.block = kTileBlock;
+    const int nx = grid.nc[0];
+    const int span_x = nx >= 3 ? 1 : 0;
+    int nrows = 1;
+    for (int d = 1; d < ndim; ++d) nrows *= grid.nc[d] >= 3 ? 3 : 1;
+    const double rho = (double)N / (double)grid.total;   // particles per cell
+    const int hc_max = 1024;
+    const int t_max = hc_max / nrows - 2 * span_x;
+    int T = (int)(kTileBlock / (rho > 1e-9 ? rho : 1e-9));
+    if (T < 1) T = 1;
+    if (T > t_max) T = t_max;
+    if (T > nx) T = nx;
+    if (T < 1) return false;
+    p.tile_len = T;
+    p.tiles_per_row = (nx + T - 1) / T;
+    long long rows = grid.total / nx;
+    p.tiles = rows * p.tiles_per_row;
+    p.hc_cap = nrows * (T + 2 * span_x);
+    // staged halo: expected size with margin, within 64 KiB of LDS
+    const double expect = rho * p.hc_cap;
+    p.list_cap = ndim == 2 ? 24 : 32;
+    const int per = ndim * 8 + 8 + 4 + (use_rk ? 8 : 0);
+    const int fixed_bytes = tile_layout(ndim, p.hc_cap, 0, p.list_cap, kTileBlock, use_rk).bytes;
+    int h = (int)(expect * 1.5) + 64;
+    const int h_lim = (65536 - 1024 - fixed_bytes) / per;
+    if (h > h_lim) h = h_lim;
+    if (h < 1) return false;
+    p.h_cap = h;
+    p.lds_bytes = tile_layout(ndim, p.hc_cap, p.h_cap, p.list_cap, kTileBlock, use_rk).bytes;
+    // persistent workgroups: as many as are resident at once
+    const long long resident = tile_grid(ndim, p.lds_bytes);
+    p.blocks = p.tiles < resident ? p.tiles : resident;
+    return p.blocks > 0;
+}
+
+int launch_step_tile(const StepArgs& a0, const TilePlan& p, hipStream_t s)
+{
+    if (a0.g.N <= 0) return 0;
+    StepArgs a = a0;
+    a.tile_len = p.tile_len;
+    a.tiles_per_row = p.tiles_per_row;
+    a.hc_cap = p.hc_cap;
+    a.h_cap = p.h_cap;
+    a.list_cap = p.list_cap;
+    const unsigned g = (unsigned)p.blocks;
+    switch (a.g.ndim) {
+        case 2: k_step_tile<2, kTileBlock><<<g, kTileBlock, p.lds_bytes, s>>>(a, p.tiles); break;
+        case 3: k_step_tile<3, kTileBlock><<<g, kTileBlock, p.lds_bytes, s>>>(a, p.tiles); break;
+        default: return 2;
+    }
+    k_fold<<<1, kFoldBlock, 0, s>>>(a.partials, g, a.result);
+    return check_launch();
+}
+
 int launch_step_list(const StepArgs& a, hipStream_t s)
 {
     if (a.g.N <= 0) return 0;
     const unsigned g = grid_for(a.g.N);
     RCP_DISPATCH_ND(a.g.ndim, k_step_list, <<<g, kBlock, 0, s>>>(a, nullptr));
+    k_fold<<<1, kFoldBlock, 0, s>>>(a.partials, g, a.result);
     return check_launch();
 }
 
@@ -1169,6 +1637,7 @@ int launch_step_direct(const StepArgs& a, hipStream_t s)
     if (a.g.N <= 0) return 0;
     const unsigned g = grid_for(a.g.N, kDirectBlock);
     RCP_DISPATCH_ND(a.g.ndim, k_step_direct, <<<g, kDirectBlock, 0, s>>>(a));
+    k_fold<<<1, kFoldBlock, 0, s>>>(a.partials, g, a.result);
     return check_launch();
 }
 
@@ -1178,6 +1647,7 @@ int launch_exact_step(const StepArgs& a, uint32_t* ref_count, uint32_t* ref_star
     if (a.g.N <= 0) return 0;
     const unsigned g = grid_for(a.g.N);
     RCP_DISPATCH_ND(a.g.ndim, k_step_list, <<<g, kBlock, 0, s>>>(a, ref_count));
+    k_fold<<<1, kFoldBlock, 0, s>>>(a.partials, g, a.result);
     if (launch_scan_u32(ref_count, ref_start, (size_t)a.g.N, scan_tmp, s)) return 1;
     k_exact_compact<<<g, kBlock, 0, s>>>(a, ref_start, ref_terms);
     RCP_DISPATCH_ND(a.g.ndim, k_exact_reduce, <<<1, kWave, 0, s>>>(a, ref_start, ref_terms));
@@ -1194,6 +1664,13 @@ int launch_rebin(BinArgs& a, void* scan_tmp, hipStream_t s)
     k_place<<<grid_for(N), kBlock, 0, s>>>(a);
     k_cell_order<<<grid_for(total), kBlock, 0, s>>>(a);
     RCP_DISPATCH_ND(a.g.ndim, k_permute, <<<grid_for(N), kBlock, 0, s>>>(a));
+    return check_launch();
+}
+
+int launch_inv_tag(const uint32_t* tag, uint32_t* inv_tag, int N, hipStream_t s)
+{
+    if (N <= 0) return 0;
+    k_inv_tag<<<grid_for(N), kBlock, 0, s>>>(tag, inv_tag, N);
     return check_launch();
 }
 
@@ -1411,6 +1888,7 @@ int launch_mean_force(const StepArgs& a, const double* F, int exact, hipStream_t
 {
     if (a.g.N <= 0) return 0;
     k_mean_force<<<grid_for(a.g.N), kBlock, 0, s>>>(a, F);
+    k_fold<<<1, kFoldBlock, 0, s>>>(a.partials, grid_for(a.g.N), a.result);
     if (exact) k_mean_force_exact<<<1, kWave, 0, s>>>(a);
     return check_launch();
 }

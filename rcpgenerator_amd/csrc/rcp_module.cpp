@@ -336,8 +336,10 @@ public:
         rcp_result_info i;
         int rc = rcp_engine_info(h_.e, &i);
         if (rc) raise_rc(rc);
-        uint64_t builds = 0, entries = 0, reorders = 0, direct = 0, overflow = 0;
-        rcp_engine_stats(h_.e, &builds, &entries, &reorders, &direct, &overflow);
+        uint64_t builds = 0, entries = 0, reorders = 0, direct = 0, overflow = 0, tsteps = 0,
+                 trebins = 0;
+        rcp_engine_stats(h_.e, &builds, &entries, &reorders, &direct, &overflow, &tsteps,
+                         &trebins);
         py::dict d;
         d["N"] = i.N;
         d["ndim"] = i.ndim;
@@ -351,6 +353,8 @@ public:
         d["reorders"] = reorders;
         d["direct_steps"] = direct;
         d["overflow_steps"] = overflow;
+        d["tile_steps"] = tsteps;
+        d["tile_rebins"] = trebins;
         double sm = 0, bm = 0;
         uint64_t sn = 0, bn = 0;
         rcp_engine_timing(h_.e, &sm, &sn, &bm, &bn);

@@ -303,26 +303,41 @@ def test_fix_height_run_vs_oracle(gpu, oracle, max_steps):
 
 # ------------------------------------------------- neighbour-mode agreement
 
-@pytest.mark.parametrize("nd,phi", [(2, 1.2), (3, 0.9), (2, 9.0)])
-def test_cell_scan_matches_lists(gpu, nd, phi):
+def clustered_config(rng, N, nd, phi, frac=0.1, side=0.06):
+    """Uniform background plus a dense cluster: tiles over the cluster hold
+    far more particles than the mean density predicts."""
+    x, D, box = random_config(rng, N, nd, phi)
+    k = int(frac * N)
+    x[:k] = 0.3 + side * rng.random((k, nd))
+    return x, D, box
+
+
+@pytest.mark.parametrize("nd,phi,cluster", [(2, 1.2, False), (3, 0.9, False), (2, 9.0, False),
+                                            (3, 4.0, False), (2, 0.5, True), (3, 0.3, True)])
+def test_neighbour_modes_agree(gpu, nd, phi, cluster):
     """With the diameters fixed no scalar feeds back into the positions, so
-    the per-step cell scan and the neighbour lists must give bit-identical
-    trajectories; phi 9 in 2-D overflows the scan's contact buffer."""
-    rng = np.random.default_rng(int(10 * phi) + nd)
-    x, D, box = random_config(rng, 20000 if phi < 5 else 3000, nd, phi)
+    the neighbour lists, the per-step cell scan and the tiled kernel must
+    give bit-identical trajectories.  Dense cases overflow the contact
+    buffers (selection path); clustered cases overflow the tiles' LDS
+    staging (global-memory path)."""
+    rng = np.random.default_rng(int(10 * phi) + nd + 100 * cluster)
+    N = 20000 if phi < 3 else 3000
+    x, D, box = (clustered_config if cluster else random_config)(rng, N, nd, phi)
     out = {}
-    for mode in (1, 2):
+    for mode in (1, 2, 3):
         e = gpu.Engine(x, D, box=box, max_steps=40, fix_diameter=True, neighbor_mode=mode)
         e.run(0)
         out[mode] = (e.result(), e.stats())
-    (a, sa), (b, sb) = out[1], out[2]
+    (a, sa), (b, sb), (c, sc) = out[1], out[2], out[3]
     assert sb["direct_steps"] == 40 and sa["direct_steps"] == 0
+    assert sc["tile_steps"] == 40
     assert same(a["positions"], b["positions"])
-    if phi > 5:
-        assert sb["overflow_steps"] > 0
+    assert same(a["positions"], c["positions"])
+    if phi > 3:
+        assert sb["overflow_steps"] > 0 and sc["overflow_steps"] > 0
 
 
-def test_auto_mode_uses_both_paths(gpu):
+def test_auto_mode_tiles_2d(gpu):
     import rcpgenerator_amd as rcp
     init = rcp.initialize_particles({"phi": 0.1, "N": 50000, "Ndim": 2, "box": [1.0, 1.0],
                                      "walls": [0, 0], "seed": 123,
@@ -332,7 +347,23 @@ def test_auto_mode_uses_both_paths(gpu):
                    max_steps=6000)
     e.run(0)
     r, st = e.result(), e.stats()
-    assert st["direct_steps"] > 0 and st["list_builds"] > 1
+    assert st["tile_steps"] == r["steps"] == 6000
+    assert st["tile_rebins"] < r["steps"]      # the skin lets binnings live
+    assert np.all(np.isfinite(r["positions"]))
+    assert np.all((r["positions"] >= 0) & (r["positions"] < 1))
+    assert np.all(np.isfinite(r["phi_history"]))
+
+
+def test_auto_mode_4d_switches(gpu):
+    import rcpgenerator_amd as rcp
+    init = rcp.initialize_particles({"phi": 0.1, "N": 20000, "Ndim": 4, "box": [1.0] * 4,
+                                     "walls": [0] * 4, "seed": 7,
+                                     "dist": {"type": "lognormal", "mu": 0.0, "sigma": 0.2}})
+    e = gpu.Engine(init["positions"], init["diameters"], box=init["box"], walls=init["walls"],
+                   max_steps=3000)
+    e.run(0)
+    r, st = e.result(), e.stats()
+    assert st["direct_steps"] > 0 and st["list_builds"] > 1 and st["tile_steps"] == 0
     assert np.all(np.isfinite(r["positions"]))
     assert np.all((r["positions"] >= 0) & (r["positions"] < 1))
     assert np.all(np.isfinite(r["phi_history"]))
