@@ -647,7 +647,11 @@ Engine::Scalars Engine::step_device(bool do_adam, double alpha, double inv_b1, d
     a.inv_tag = S.inv_tag;
     a.partials = S.partials;
     a.result = S.result;
-    if (mode == Mode::Tile) S.ensure_partials((size_t)tile_plan_.blocks + 1);
+    if (mode == Mode::Tile) {
+        S.ensure_partials((size_t)tile_plan_.blocks + 1);
+        a.dmax = dmax_now();
+        a.pad = 0.5 * tile_skin_;
+    }
     if (opt_.exact_reductions) S.ensure_inv_tag();
     a.partials = S.partials;
     begin_timer();

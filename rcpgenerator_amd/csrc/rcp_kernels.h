@@ -80,8 +80,11 @@ struct StepArgs {
     const uint32_t* inv_tag = nullptr; // N: working position of reference index
     double* partials = nullptr;      // gridDim * S_NUM rows, folded into result
     double* result = nullptr;        // S_NUM totals
-    // tiled step (launch_step_tile): plan from plan_tiles
+    // tiled step (launch_step_tile): plan from plan_tiles; dmax = largest
+    // current diameter, pad = largest drift of a particle from its binned
+    // cell (half the skin)
     int tile_len = 0, tiles_per_row = 0, hc_cap = 0, h_cap = 0, list_cap = 0;
+    double dmax = 0.0, pad = 0.0;
 };
 
 // Tiled step over the binned working order (ndim 2 and 3).  One workgroup

@@ -587,107 +587,105 @@ __device__ __forceinline__ int find_cell(const uint32_t* off, int lo, int hi, ui
     return lo;
 }
 
-// Staged (LDS) or global view of a tile's particles: slot j is an LDS
-// index when LDS is true, a working position otherwise.
-template <int ND, bool LDS>
-struct TileSrc {
-    const StepArgs* a;
-    const double* sx;
-    const double* sD;
-    const double* srk;
-    const uint32_t* stag;
-    int h_cap;
-    bool rk;
-    __device__ __forceinline__ void get(uint32_t j, double (&xj)[ND], double& Dj,
-                                        double& rkj) const
-    {
-        if (LDS) {
-#pragma unroll
-            for (int d = 0; d < ND; ++d) xj[d] = sx[d * h_cap + j];
-            Dj = sD[j];
-            rkj = rk ? srk[j] : 0.0;
-        } else {
-            load_partner<ND>(*a, j, xj, Dj, rkj);
-        }
-    }
-    __device__ __forceinline__ void pos(uint32_t j, double (&xj)[ND], double& Dj) const
-    {
-        if (LDS) {
-#pragma unroll
-            for (int d = 0; d < ND; ++d) xj[d] = sx[d * h_cap + j];
-            Dj = sD[j];
-        } else {
-            double rkj;
-            load_partner<ND>(*a, j, xj, Dj, rkj);
-        }
-    }
-    __device__ __forceinline__ uint32_t tag(uint32_t j) const { return LDS ? stag[j] : a->tag[j]; }
+// Tile geometry shared by the staging and the per-particle passes.
+template <int ND>
+struct TileGeo {
+    int nx, span_x, cx_lo, ncx, kx, nrows, ro_c;
+    int rc[ND], rspan[ND];
+    double e[ND];   // cell edge per axis (box / cells)
 };
 
-// Forces, pair terms and the ADAM tail of the tile's own particles.
-template <int ND, int BLOCK, bool LDS>
-__device__ __forceinline__ void tile_particles(const StepArgs& a, const TileSrc<ND, LDS>& src,
-                                               const uint32_t* hc_first, const uint32_t* hc_off,
-                                               uint16_t* mylist, int nrows, int kx, int span_x,
-                                               int own_h0, int ncx, uint32_t own_lo,
-                                               uint32_t own_hi, uint32_t q_own,
-                                               double (&acc)[S_NUM])
+// Forces, pair terms and the ADAM tail of the tile's own particles, from
+// the staged halo.  PLAIN tiles (no halo cell across the periodic boundary,
+// seven or more cells on every axis) have |x_j - x_i| < L/2 for every pair
+// in a stencil, so the minimum image is the identity and the plain
+// difference is the reference's arithmetic bit for bit; they also skip
+// the stencil cells a particle cannot reach ((Di + Dmax)/2 plus drift).
+// Other tiles take the full minimum-image form over the whole stencil.
+template <int ND, int BLOCK, bool PLAIN>
+__device__ __forceinline__ void tile_particles_lds(
+    const StepArgs& a, const TileGeo<ND>& t, const uint32_t* hc_off, const double* sx,
+    const double* sD, const double* srk, const uint32_t* stag, uint16_t* mylist, int own_h0,
+    uint32_t own_lo, uint32_t own_hi, uint32_t q_own, double (&acc)[S_NUM])
 {
     const int cap = a.list_cap;
+    const int h_cap = a.h_cap;
+    const bool rk = a.use_rk != 0;
+    const double tol = 1e-9;
     for (uint32_t i = own_lo + threadIdx.x; i < own_hi; i += BLOCK) {
         const uint32_t q = q_own + (i - own_lo);
         Own<ND> o;
         load_own<ND>(a, (int)q, o);
-        const int kc = find_cell(hc_off, own_h0, own_h0 + ncx, i) - (own_h0 - span_x);
-        const uint32_t self = LDS ? i : q;
+        const int kc = find_cell(hc_off, own_h0, own_h0 + t.ncx, i) - (own_h0 - t.span_x);
+        int k0 = kc - t.span_x, k1 = kc + t.span_x;
+        double R = 0.0;
+        if (PLAIN) {
+            // halo cell k along x is global cell cx_lo - 1 + k: [cx*e, (cx+1)*e)
+            R = 0.5 * (o.D + a.dmax) + a.pad;
+            const double left = (double)(t.cx_lo - 1 + kc) * t.e[0];
+            if (o.x[0] - R > left * (1.0 + tol) + tol * t.e[0]) k0 = kc;
+            if (o.x[0] + R < (left + t.e[0]) * (1.0 - tol) - tol * t.e[0]) k1 = kc;
+        }
         double F[ND];
 #pragma unroll
         for (int d = 0; d < ND; ++d) F[d] = 0.0;
-        int n = 0;
-        auto push = [&](uint32_t j) {
-            if (n < cap) mylist[n * BLOCK] = (uint16_t)j;
-            ++n;
+        auto hit = [&](uint32_t j) {
+            double d2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                double dd = sx[d * h_cap + j] - o.x[d];
+                if (!PLAIN) dd = mic(dd, a.g.inv_box[d], a.g.box[d]);
+                d2 += dd * dd;
+            }
+            const double r = 0.5 * (o.D + sD[j]);
+            return d2 < r * r;
         };
-        if (LDS) {
-            // the x-neighbour cells of a row are adjacent in LDS: one range
-            // per row, four candidates per pass so their reads overlap
-            for (int ro = 0; ro < nrows; ++ro) {
-                const int hb = ro * kx + kc;
-                const uint32_t j1 = hc_off[hb + span_x + 1];
-                uint32_t j = hc_off[hb - span_x];
-                for (; j + 4 <= j1; j += 4) {
-                    double xj[4][ND], Dj[4];
+        int n = 0;
+        for (int ro = 0; ro < t.nrows; ++ro) {
+            if (PLAIN) {
+                // rows out of reach along y (and z) are skipped
+                bool reach = true;
+                int tt = ro;
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) src.pos(j + u, xj[u], Dj[u]);
-#pragma unroll
-                    for (int u = 0; u < 4; ++u)
-                        if (j + u != self && overlap_at<ND>(a, o, xj[u], Dj[u])) push(j + u);
+                for (int d = 1; d < ND; ++d) {
+                    const int kd = tt % 3;
+                    tt /= 3;
+                    const double lo = (double)(t.rc[d] + kd - 1) * t.e[d];
+                    if (kd == 0 && o.x[d] - R > (lo + t.e[d]) * (1.0 + tol) + tol * t.e[d])
+                        reach = false;
+                    if (kd == 2 && o.x[d] + R < lo * (1.0 - tol) - tol * t.e[d]) reach = false;
                 }
-                for (; j < j1; ++j) {
-                    double xj[ND], Dj;
-                    src.pos(j, xj, Dj);
-                    if (j != self && overlap_at<ND>(a, o, xj, Dj)) push(j);
-                }
+                if (!reach) continue;
             }
-        } else {
-            for (int ro = 0; ro < nrows; ++ro) {
-                for (int dk = -span_x; dk <= span_x; ++dk) {
-                    const int h = ro * kx + kc + dk;
-                    const uint32_t j0 = hc_first[h], cnt = hc_off[h + 1] - hc_off[h];
-                    for (uint32_t j = j0; j < j0 + cnt; ++j) {
-                        double xj[ND], Dj;
-                        src.pos(j, xj, Dj);
-                        if (j != self && overlap_at<ND>(a, o, xj, Dj)) push(j);
+            const uint32_t j1 = hc_off[ro * t.kx + k1 + 1];
+            uint32_t j = hc_off[ro * t.kx + k0];
+            for (; j + 4 <= j1; j += 4) {
+                bool h4[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) h4[u] = hit(j + u);
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (h4[u] && j + u != i) {
+                        if (n < cap) mylist[n * BLOCK] = (uint16_t)(j + u);
+                        ++n;
                     }
-                }
             }
+            for (; j < j1; ++j)
+                if (j != i && hit(j)) {
+                    if (n < cap) mylist[n * BLOCK] = (uint16_t)j;
+                    ++n;
+                }
         }
-        if (n <= cap && (LDS || n == 0)) {
+        auto pos = [&](uint32_t j, double (&xj)[ND]) {
+#pragma unroll
+            for (int d = 0; d < ND; ++d) xj[d] = sx[d * h_cap + j];
+        };
+        if (n <= cap) {
             for (int p = 1; p < n; ++p) {
                 const uint16_t jk = mylist[p * BLOCK];
-                const uint32_t tk = src.tag(jk);
+                const uint32_t tk = stag[jk];
                 int k = p;
-                while (k > 0 && src.tag(mylist[(k - 1) * BLOCK]) > tk) {
+                while (k > 0 && stag[mylist[(k - 1) * BLOCK]] > tk) {
                     mylist[k * BLOCK] = mylist[(k - 1) * BLOCK];
                     --k;
                 }
@@ -695,41 +693,114 @@ __device__ __forceinline__ void tile_particles(const StepArgs& a, const TileSrc<
             }
             for (int p = 0; p < n; ++p) {
                 const uint32_t j = mylist[p * BLOCK];
-                double xj[ND], Dj, rkj;
-                src.get(j, xj, Dj, rkj);
-                pair_contact<ND>(a, o, xj, Dj, rkj, src.tag(j), F, acc, nullptr);
+                double xj[ND];
+                pos(j, xj);
+                pair_contact<ND>(a, o, xj, sD[j], rk ? srk[j] : 0.0, stag[j], F, acc, nullptr);
             }
         } else {
-            // more contacts than the buffer (or positions beyond 16 bits in
-            // the global view): take them in tag order by repeated scans
-            if (n > cap) acc[S_OVERFLOW] += 1.0;
+            // more contacts than the buffer: take them in tag order by
+            // repeated scans of the full stencil
+            acc[S_OVERFLOW] += 1.0;
             long long last = -1;
             while (true) {
                 unsigned long long best = ~0ull;
-                for (int ro = 0; ro < nrows; ++ro) {
-                    for (int dk = -span_x; dk <= span_x; ++dk) {
-                        const int h = ro * kx + kc + dk;
-                        const uint32_t j0 = LDS ? hc_off[h] : hc_first[h];
-                        const uint32_t cnt = hc_off[h + 1] - hc_off[h];
-                        for (uint32_t j = j0; j < j0 + cnt; ++j) {
-                            if (j == self) continue;
-                            const uint32_t tj = src.tag(j);
-                            if ((long long)tj <= last) continue;
-                            const unsigned long long key = ((unsigned long long)tj << 32) | j;
-                            if (key >= best) continue;
-                            double xj[ND], Dj;
-                            src.pos(j, xj, Dj);
-                            if (overlap_at<ND>(a, o, xj, Dj)) best = key;
-                        }
+                for (int ro = 0; ro < t.nrows; ++ro) {
+                    const uint32_t j1 = hc_off[ro * t.kx + kc + t.span_x + 1];
+                    for (uint32_t j = hc_off[ro * t.kx + kc - t.span_x]; j < j1; ++j) {
+                        if (j == i) continue;
+                        const uint32_t tj = stag[j];
+                        if ((long long)tj <= last) continue;
+                        const unsigned long long key = ((unsigned long long)tj << 32) | j;
+                        if (key >= best) continue;
+                        double xj[ND];
+                        pos(j, xj);
+                        if (overlap_at<ND>(a, o, xj, sD[j])) best = key;
                     }
                 }
                 if (best == ~0ull) break;
                 const uint32_t j = (uint32_t)(best & 0xffffffffull);
-                double xj[ND], Dj, rkj;
-                src.get(j, xj, Dj, rkj);
-                pair_contact<ND>(a, o, xj, Dj, rkj, (uint32_t)(best >> 32), F, acc, nullptr);
+                double xj[ND];
+                pos(j, xj);
+                pair_contact<ND>(a, o, xj, sD[j], rk ? srk[j] : 0.0, (uint32_t)(best >> 32), F,
+                                 acc, nullptr);
                 last = (long long)(best >> 32);
             }
+        }
+        particle_tail<ND>(a, (int)q, o, F, acc);
+    }
+}
+
+// The same from global memory, for a tile whose halo exceeds the LDS budget:
+// every candidate is tested with the reference's arithmetic directly.
+// Contacts are buffered as working positions in the (unused) staging area,
+// cap per thread; beyond that, repeated scans take them in tag order.
+template <int ND, int BLOCK>
+__device__ __forceinline__ void tile_particles_global(
+    const StepArgs& a, const TileGeo<ND>& t, const uint32_t* hc_first, const uint32_t* hc_off,
+    int own_h0, uint32_t own_lo, uint32_t own_hi, uint32_t q_own, uint32_t* buf, int cap,
+    double (&acc)[S_NUM])
+{
+    uint32_t* mylist = buf + threadIdx.x;
+    for (uint32_t i = own_lo + threadIdx.x; i < own_hi; i += BLOCK) {
+        const uint32_t q = q_own + (i - own_lo);
+        Own<ND> o;
+        load_own<ND>(a, (int)q, o);
+        const int kc = find_cell(hc_off, own_h0, own_h0 + t.ncx, i) - (own_h0 - t.span_x);
+        double F[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) F[d] = 0.0;
+        int n = 0;
+        for (int ro = 0; ro < t.nrows; ++ro) {
+            for (int dk = -t.span_x; dk <= t.span_x; ++dk) {
+                const int h = ro * t.kx + kc + dk;
+                const uint32_t j0 = hc_first[h], cnt = hc_off[h + 1] - hc_off[h];
+                for (uint32_t j = j0; j < j0 + cnt; ++j) {
+                    if (j == q || !overlaps<ND>(a, o, j)) continue;
+                    if (n < cap) mylist[n * BLOCK] = j;
+                    ++n;
+                }
+            }
+        }
+        if (n <= cap) {
+            for (int p = 1; p < n; ++p) {
+                const uint32_t jk = mylist[p * BLOCK];
+                const uint32_t tk = a.tag[jk];
+                int k = p;
+                while (k > 0 && a.tag[mylist[(k - 1) * BLOCK]] > tk) {
+                    mylist[k * BLOCK] = mylist[(k - 1) * BLOCK];
+                    --k;
+                }
+                mylist[k * BLOCK] = jk;
+            }
+            for (int p = 0; p < n; ++p) {
+                const uint32_t j = mylist[p * BLOCK];
+                contact<ND>(a, o, j, a.tag[j], F, acc, nullptr);
+            }
+            particle_tail<ND>(a, (int)q, o, F, acc);
+            continue;
+        }
+        acc[S_OVERFLOW] += 1.0;
+        long long last = -1;
+        while (true) {
+            unsigned long long best = ~0ull;
+            for (int ro = 0; ro < t.nrows; ++ro) {
+                for (int dk = -t.span_x; dk <= t.span_x; ++dk) {
+                    const int h = ro * t.kx + kc + dk;
+                    const uint32_t j0 = hc_first[h], cnt = hc_off[h + 1] - hc_off[h];
+                    for (uint32_t j = j0; j < j0 + cnt; ++j) {
+                        if (j == q) continue;
+                        const uint32_t tj = a.tag[j];
+                        if ((long long)tj <= last) continue;
+                        const unsigned long long key = ((unsigned long long)tj << 32) | j;
+                        if (key >= best) continue;
+                        if (overlaps<ND>(a, o, j)) best = key;
+                    }
+                }
+            }
+            if (best == ~0ull) break;
+            contact<ND>(a, o, (uint32_t)(best & 0xffffffffull), (uint32_t)(best >> 32), F, acc,
+                        nullptr);
+            last = (long long)(best >> 32);
         }
         particle_tail<ND>(a, (int)q, o, F, acc);
     }
@@ -756,6 +827,12 @@ __global__ __launch_bounds__(BLOCK) void k_step_tile(StepArgs a, long long ntile
 #pragma unroll
     for (int k = 0; k < S_NUM; ++k) acc[k] = 0.0;
 
+    TileGeo<ND> t;
+    t.nx = a.grid.nc[0];
+    t.span_x = t.nx >= 3 ? 1 : 0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) t.e[d] = a.g.box[d] / (double)a.grid.nc[d];
+
     // Workgroups are dealt to the XCDs round-robin; the workgroups of one XCD
     // take one contiguous band of tiles, so neighbouring tiles (which share
     // halo cells) hit the same L2.
@@ -767,89 +844,96 @@ __global__ __launch_bounds__(BLOCK) void k_step_tile(StepArgs a, long long ntile
     const long long t_begin = (long long)xcd * band;
     const long long t_end = (t_begin + band < ntiles) ? t_begin + band : ntiles;
     for (long long tix = t_begin + lane_blk; tix < t_end; tix += per_xcd) {
-    __syncthreads();   // LDS of the previous tile is free
+        __syncthreads();   // LDS of the previous tile is free
 
-    // ---- the tile and its halo cells
-    const int nx = a.grid.nc[0];
-    const int span_x = nx >= 3 ? 1 : 0;
-    const int tile = (int)(tix % a.tiles_per_row);
-    const int row = (int)(tix / a.tiles_per_row);
-    const int cx_lo = tile * a.tile_len;
-    const int cx_hi = (cx_lo + a.tile_len < nx) ? cx_lo + a.tile_len : nx;
-    const int ncx = cx_hi - cx_lo;
-    const int kx = ncx + 2 * span_x;
-    int rc[ND], rspan[ND];
-    int nrows = 1, ro_c = 0;
-    {
-        int r = row;
+        // ---- the tile and its halo cells
+        const int tile = (int)(tix % a.tiles_per_row);
+        const int row = (int)(tix / a.tiles_per_row);
+        t.cx_lo = tile * a.tile_len;
+        const int cx_hi = (t.cx_lo + a.tile_len < t.nx) ? t.cx_lo + a.tile_len : t.nx;
+        t.ncx = cx_hi - t.cx_lo;
+        t.kx = t.ncx + 2 * t.span_x;
+        t.nrows = 1;
+        t.ro_c = 0;
+        {
+            int r = row;
 #pragma unroll
-        for (int d = 1; d < ND; ++d) {
-            rc[d] = r % a.grid.nc[d];
-            r /= a.grid.nc[d];
-            rspan[d] = a.grid.nc[d] >= 3 ? 3 : 1;
-            if (rspan[d] == 3) ro_c += nrows;
-            nrows *= rspan[d];
-        }
-    }
-    const int HC = nrows * kx;
-    for (int h = tid; h < HC; h += BLOCK) {
-        const int ro = h / kx, k = h - ro * kx;
-        int cx = cx_lo - span_x + k;
-        if (cx < 0) cx += nx;
-        if (cx >= nx) cx -= nx;
-        long long cc = cx, stride = nx;
-        int t = ro;
-#pragma unroll
-        for (int d = 1; d < ND; ++d) {
-            int cd = rc[d];
-            if (rspan[d] == 3) {
-                cd += t % 3 - 1;
-                t /= 3;
-                if (cd < 0) cd += a.grid.nc[d];
-                if (cd >= a.grid.nc[d]) cd -= a.grid.nc[d];
+            for (int d = 1; d < ND; ++d) {
+                t.rc[d] = r % a.grid.nc[d];
+                r /= a.grid.nc[d];
+                t.rspan[d] = a.grid.nc[d] >= 3 ? 3 : 1;
+                if (t.rspan[d] == 3) t.ro_c += t.nrows;
+                t.nrows *= t.rspan[d];
             }
-            cc += (long long)cd * stride;
-            stride *= a.grid.nc[d];
         }
-        const uint32_t s0 = a.cell_start[cc];
-        hc_first[h] = s0;
-        hc_off[h] = a.cell_start[cc + 1] - s0;
-    }
-    __syncthreads();
-    block_scan_inplace<BLOCK>(hc_off, HC);
-    const uint32_t H = hc_off[HC];
-    const bool lds = H <= (uint32_t)h_cap;
-    const int own_h0 = ro_c * kx + span_x;
-    const uint32_t own_lo = hc_off[own_h0], own_hi = hc_off[own_h0 + ncx];
-    const uint32_t q_own = hc_first[own_h0];
-
-    // ---- stage the halo (coalesced: cells of a row are contiguous)
-    if (lds && own_hi > own_lo) {
-        for (uint32_t i = tid; i < H; i += BLOCK) {
-            const int h = find_cell(hc_off, 0, HC, i);
-            const uint32_t p = hc_first[h] + (i - hc_off[h]);
+        const int HC = t.nrows * t.kx;
+        for (int h = tid; h < HC; h += BLOCK) {
+            const int ro = h / t.kx, k = h - ro * t.kx;
+            int cx = t.cx_lo - t.span_x + k;
+            if (cx < 0) cx += t.nx;
+            if (cx >= t.nx) cx -= t.nx;
+            long long cc = cx, stride = t.nx;
+            int tt = ro;
 #pragma unroll
-            for (int d = 0; d < ND; ++d) sx[d * h_cap + i] = a.x_in[(size_t)p * ND + d];
-            const bool fp = a.fixed && a.fixed[p];
-            const double D0p = a.D0[p];
-            const double Dp = fp ? D0p : D0p * a.kappa;
-            sD[i] = Dp;
-            if (rk_lds) srk[i] = a.rk ? a.rk[p] : (fp ? 0.0 : Dp * 0.5);
-            stag[i] = a.tag[p];
+            for (int d = 1; d < ND; ++d) {
+                int cd = t.rc[d];
+                if (t.rspan[d] == 3) {
+                    cd += tt % 3 - 1;
+                    tt /= 3;
+                    if (cd < 0) cd += a.grid.nc[d];
+                    if (cd >= a.grid.nc[d]) cd -= a.grid.nc[d];
+                }
+                cc += (long long)cd * stride;
+                stride *= a.grid.nc[d];
+            }
+            const uint32_t s0 = a.cell_start[cc];
+            hc_first[h] = s0;
+            hc_off[h] = a.cell_start[cc + 1] - s0;
         }
-    }
-    __syncthreads();
+        __syncthreads();
+        block_scan_inplace<BLOCK>(hc_off, HC);
+        const uint32_t H = hc_off[HC];
+        const bool lds = H <= (uint32_t)h_cap;
+        const int own_h0 = t.ro_c * t.kx + t.span_x;
+        const uint32_t own_lo = hc_off[own_h0], own_hi = hc_off[own_h0 + t.ncx];
+        const uint32_t q_own = hc_first[own_h0];
+        if (own_hi == own_lo) continue;   // uniform across the workgroup
 
-    uint16_t* mylist = lists + tid;
-    if (lds) {
-        const TileSrc<ND, true> src{&a, sx, sD, srk, stag, h_cap, rk_lds};
-        tile_particles<ND, BLOCK, true>(a, src, hc_first, hc_off, mylist, nrows, kx, span_x,
-                                        own_h0, ncx, own_lo, own_hi, q_own, acc);
-    } else {
-        const TileSrc<ND, false> src{&a, sx, sD, srk, stag, h_cap, rk_lds};
-        tile_particles<ND, BLOCK, false>(a, src, hc_first, hc_off, mylist, nrows, kx, span_x,
-                                         own_h0, ncx, own_lo, own_hi, q_own, acc);
-    }
+        if (lds) {
+            // ---- stage the halo (coalesced: cells of a row are contiguous)
+            for (uint32_t i = tid; i < H; i += BLOCK) {
+                const int h = find_cell(hc_off, 0, HC, i);
+                const uint32_t p = hc_first[h] + (i - hc_off[h]);
+#pragma unroll
+                for (int d = 0; d < ND; ++d) sx[d * h_cap + i] = a.x_in[(size_t)p * ND + d];
+                const bool fp = a.fixed && a.fixed[p];
+                const double D0p = a.D0[p];
+                const double Dp = fp ? D0p : D0p * a.kappa;
+                sD[i] = Dp;
+                if (rk_lds) srk[i] = a.rk ? a.rk[p] : (fp ? 0.0 : Dp * 0.5);
+                stag[i] = a.tag[p];
+            }
+            __syncthreads();
+            // no halo cell across the periodic boundary, seven or more cells
+            // on every axis: the minimum image is the identity in this tile
+            bool plain = t.span_x && t.cx_lo >= 1 && t.cx_lo + t.ncx <= t.nx - 1 && t.nx >= 7;
+#pragma unroll
+            for (int d = 1; d < ND; ++d)
+                plain = plain && a.grid.nc[d] >= 7 && t.rc[d] >= 1 &&
+                        t.rc[d] <= a.grid.nc[d] - 2;
+            if (plain)
+                tile_particles_lds<ND, BLOCK, true>(a, t, hc_off, sx, sD, srk, stag, lists + tid,
+                                                    own_h0, own_lo, own_hi, q_own, acc);
+            else
+                tile_particles_lds<ND, BLOCK, false>(a, t, hc_off, sx, sD, srk, stag, lists + tid,
+                                                     own_h0, own_lo, own_hi, q_own, acc);
+        } else {
+            // the staging area holds ND * h_cap doubles: 2 * ND * h_cap / BLOCK
+            // u32 entries per thread
+            const int gcap = 2 * ND * h_cap / BLOCK;
+            tile_particles_global<ND, BLOCK>(a, t, hc_first, hc_off, own_h0, own_lo, own_hi,
+                                             q_own, reinterpret_cast<uint32_t*>(sx), gcap, acc);
+        }
     }   // tiles
     double blk[S_NUM];
     block_reduce<BLOCK>(acc, blk);
