@@ -234,14 +234,12 @@ void DeviceState::rebin(const Geometry& g, double reach)
     b.v = v;
     b.D0 = D0;
     b.tag = tag;
-    b.origin = origin;
     b.fixed = fixed;
     b.x_o = x_alt;
     b.m_o = m_alt;
     b.v_o = v_alt;
     b.D0_o = D0_alt;
     b.tag_o = tag_alt;
-    b.origin_o = origin_alt;
     b.fixed_o = fixed_alt;
     b.cell_of = cell_of;
     b.slot = slot;
@@ -254,7 +252,6 @@ void DeviceState::rebin(const Geometry& g, double reach)
     std::swap(v, v_alt);
     std::swap(D0, D0_alt);
     std::swap(tag, tag_alt);
-    std::swap(origin, origin_alt);
     std::swap(fixed, fixed_alt);
     binned = true;
     inv_tag_valid = false;
@@ -324,6 +321,10 @@ void DeviceState::reference_reorder(const Geometry& g)
     if (launch_reference_reorder(g, x, tag, inv_tag, tag_tmp, sort, s))
         throw HipFailure("reference reorder launch");
     inv_tag_valid = true;
+    // origin is kept in reference order: carry it to the new order
+    if (launch_rank_scatter_u64(origin, origin_alt, tag_tmp, N, s))
+        throw HipFailure("origin scatter launch");
+    std::swap(origin, origin_alt);
 }
 
 void DeviceState::to_ref(const double* src_w, int width, std::vector<double>& out)
@@ -362,8 +363,7 @@ void DeviceState::set_d0_ref(const std::vector<double>& D0h, const std::vector<i
 void DeviceState::origin_ref(std::vector<uint64_t>& o)
 {
     o.resize(N);
-    if (launch_to_ref_u64(origin, u64tmp, tag, N, s)) throw HipFailure("scatter launch");
-    hip_check(hipMemcpyAsync(o.data(), u64tmp, (size_t)N * sizeof(uint64_t),
+    hip_check(hipMemcpyAsync(o.data(), origin, (size_t)N * sizeof(uint64_t),
                              hipMemcpyDeviceToHost, s), "download");
     sync();
 }
@@ -550,20 +550,36 @@ double Engine::dmax_now() const
     return std::max(D0max_free_ * kappa_D_, D0max_fixed_);
 }
 
-void Engine::begin_timer()
+// Event pairs per timed section (0: re-binning / list build, 1: step
+// kernel).  Nothing waits on them: they are read after the step's own
+// stream synchronisation (flush_timers).
+void Engine::begin_timer(int slot)
 {
-    if (opt_.time_kernels) hip_check(hipEventRecord(ev_[0], stream_), "event");
+    if (opt_.time_kernels) hip_check(hipEventRecord(ev_[2 * slot], stream_), "event");
 }
 
-void Engine::end_timer(double& acc_ms, uint64_t& n)
+void Engine::end_timer(int slot)
 {
     if (!opt_.time_kernels) return;
-    hip_check(hipEventRecord(ev_[1], stream_), "event");
-    hip_check(hipEventSynchronize(ev_[1]), "event sync");
-    float ms = 0.0f;
-    hip_check(hipEventElapsedTime(&ms, ev_[0], ev_[1]), "event time");
-    acc_ms += ms;
-    ++n;
+    hip_check(hipEventRecord(ev_[2 * slot + 1], stream_), "event");
+    timer_pending_[slot] = true;
+}
+
+void Engine::flush_timers()
+{
+    for (int slot = 0; slot < 2; ++slot) {
+        if (!timer_pending_[slot]) continue;
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, ev_[2 * slot], ev_[2 * slot + 1]), "event time");
+        if (slot == 0) {
+            t_build_ms_ += ms;
+            ++t_build_n_;
+        } else {
+            t_step_ms_ += ms;
+            ++t_step_n_;
+        }
+        timer_pending_[slot] = false;
+    }
 }
 
 // ------------------------------------------------------- neighbour lists
@@ -575,12 +591,12 @@ void Engine::build_lists()
     const double kD = kappa_D_;
     t_list_ = shell0_ * kD;
     if (!(t_list_ > 0.0)) t_list_ = 1e-3 * std::max(dmax_now(), 1e-300);
-    begin_timer();
+    begin_timer(0);
     st_->rebin(geo_, dmax_now() + t_list_);
     st_->build_rows(geo_, kD, t_list_, 0, 0);
     hip_check(hipMemcpyAsync(st_->anchor, st_->x, N_ * nd_ * sizeof(double),
                              hipMemcpyDeviceToDevice, stream_), "anchor copy");
-    end_timer(t_build_ms_, t_build_n_);
+    end_timer(0);
     kappa_b_ = kD;
     fh_drift_ = 0.0;
     need_build_ = false;
@@ -600,14 +616,14 @@ void Engine::tile_rebin(double skin)
     // Cells at least as wide as the largest contact distance plus the skin;
     // the binning stays valid while every particle has moved less than half
     // the skin (diameter growth and box stretch included).
-    begin_timer();
+    begin_timer(0);
     st_->rebin(geo_, dmax_now() + skin);
     if (!plan_tiles((int)nd_, st_->grid, (long long)N_, has_fixed_ ? 1 : 0, tile_plan_))
         throw RtError("rcp: no tile plan for this grid");
     if (skin > 0.0)
         hip_check(hipMemcpyAsync(st_->anchor, st_->x, N_ * nd_ * sizeof(double),
                                  hipMemcpyDeviceToDevice, stream_), "anchor copy");
-    end_timer(t_build_ms_, t_build_n_);
+    end_timer(0);
     tile_skin_ = skin;
     tile_binned_ = true;
     kappa_b_ = kappa_D_;
@@ -654,23 +670,27 @@ Engine::Scalars Engine::step_device(bool do_adam, double alpha, double inv_b1, d
     }
     if (opt_.exact_reductions) S.ensure_inv_tag();
     a.partials = S.partials;
-    begin_timer();
+    begin_timer(1);
+    hipEvent_t mark = opt_.time_kernels ? ev_[3] : nullptr;
     if (mode == Mode::Tile) {
-        if (launch_step_tile(a, tile_plan_, stream_)) throw HipFailure("tile step launch");
+        if (launch_step_tile(a, tile_plan_, stream_, mark))
+            throw HipFailure("tile step launch");
     } else if (direct) {
-        if (launch_step_direct(a, stream_)) throw HipFailure("direct step launch");
+        if (launch_step_direct(a, stream_, mark)) throw HipFailure("direct step launch");
     } else if (opt_.exact_reductions) {
         a.mag_out = S.mag;
         a.contrib = S.contrib;
-        if (launch_exact_step(a, S.ref_count, S.ref_start, S.ref_terms, S.scan_tmp, stream_))
+        if (launch_exact_step(a, S.ref_count, S.ref_start, S.ref_terms, S.scan_tmp, stream_,
+                              mark))
             throw HipFailure("exact step launch");
     } else {
-        if (launch_step_list(a, stream_)) throw HipFailure("list step launch");
+        if (launch_step_list(a, stream_, mark)) throw HipFailure("list step launch");
     }
-    end_timer(t_step_ms_, t_step_n_);
+    if (opt_.time_kernels) timer_pending_[1] = true;   // mark was recorded after the kernel
     hip_check(hipMemcpyAsync(S.h_result, S.result, S_NUM * sizeof(double), hipMemcpyDeviceToHost,
                              stream_), "scalars download");
     hip_check(hipStreamSynchronize(stream_), "step sync");
+    flush_timers();
     Scalars s;
     for (int k = 0; k < S_NUM; ++k) s.v[k] = S.h_result[k];
     if (do_adam) std::swap(S.x, S.x_alt);
@@ -978,9 +998,9 @@ bool Engine::step_once()
     if (mode_ == Mode::Tile) {
         ++tile_steps_;
     } else if (direct) {
-        begin_timer();
+        begin_timer(0);
         st_->rebin(geo_, dmax_now());
-        end_timer(t_build_ms_, t_build_n_);
+        end_timer(0);
         need_build_ = true;     // rows are gone with the re-binning
         ++direct_run_;
         ++direct_steps_total_;

@@ -87,6 +87,7 @@ public:
            *v_alt = nullptr, *D0 = nullptr, *D0_alt = nullptr, *anchor = nullptr,
            *tmp = nullptr, *tmp2 = nullptr, *rk = nullptr;
     uint32_t *tag = nullptr, *tag_alt = nullptr, *inv_tag = nullptr, *tag_tmp = nullptr;
+    // origin (input index) in reference order, not working order
     unsigned long long *origin = nullptr, *origin_alt = nullptr, *u64tmp = nullptr;
     int8_t *fixed = nullptr, *fixed_alt = nullptr, *i8tmp = nullptr;
     // binning / rows
@@ -182,8 +183,9 @@ private:
     bool step_once();
     int record(std::size_t step, bool capture, bool final_sample, rcp_observer_fn fn,
                void* user);
-    void begin_timer();
-    void end_timer(double& acc_ms, uint64_t& n);
+    void begin_timer(int slot);
+    void end_timer(int slot);
+    void flush_timers();
 
     // configuration
     std::size_t N_ = 0, nd_ = 0;
@@ -234,7 +236,8 @@ private:
              overflow_steps_ = 0;
 
     // timing (HIP events on stream_)
-    hipEvent_t ev_[2] = {nullptr, nullptr};
+    hipEvent_t ev_[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool timer_pending_[2] = {false, false};
     double t_step_ms_ = 0.0, t_build_ms_ = 0.0;
     uint64_t t_step_n_ = 0, t_build_n_ = 0;
 

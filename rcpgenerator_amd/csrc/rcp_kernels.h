@@ -102,17 +102,21 @@ struct TilePlan {
 };
 // false when the tiled step does not handle this shape (ndim not 2 or 3)
 bool plan_tiles(int ndim, const CellGrid& grid, long long N, int use_rk, TilePlan& p);
-int launch_step_tile(const StepArgs& a, const TilePlan& p, hipStream_t s);
+int launch_step_tile(const StepArgs& a, const TilePlan& p, hipStream_t s,
+                     hipEvent_t mark = nullptr);
 
 // blocks of the widest step launch over n particles (partials: +1 rows)
 size_t step_partial_blocks(size_t n);
-int launch_step_list(const StepArgs& a, hipStream_t s);
-int launch_step_direct(const StepArgs& a, hipStream_t s);
+// Each step launcher runs the step kernel and then k_fold; when `mark` is
+// given it is recorded between the two (timing of the step kernel alone).
+int launch_step_list(const StepArgs& a, hipStream_t s, hipEvent_t mark = nullptr);
+int launch_step_direct(const StepArgs& a, hipStream_t s, hipEvent_t mark = nullptr);
 // List step whose scalar terms are then folded in the reference's
 // one-thread order (a.contrib: 4 per row slot, a.mag_out: N, a.inv_tag).
 // ref_count / ref_start: N+1; ref_terms: 4 per recorded pair term.
 int launch_exact_step(const StepArgs& a, uint32_t* ref_count, uint32_t* ref_start,
-                      double* ref_terms, void* scan_tmp, hipStream_t s);
+                      double* ref_terms, void* scan_tmp, hipStream_t s,
+                      hipEvent_t mark = nullptr);
 
 // ---------------------------------------------------------------- binning
 // Re-binning to a new working order: old-order arrays are gathered into
@@ -127,7 +131,6 @@ struct BinArgs {
     const double* v = nullptr;
     const double* D0 = nullptr;
     const uint32_t* tag = nullptr;
-    const unsigned long long* origin = nullptr;
     const int8_t* fixed = nullptr;
     // new working order
     double* x_o = nullptr;
@@ -135,7 +138,6 @@ struct BinArgs {
     double* v_o = nullptr;
     double* D0_o = nullptr;
     uint32_t* tag_o = nullptr;
-    unsigned long long* origin_o = nullptr;
     int8_t* fixed_o = nullptr;
     // scratch
     uint32_t* cell_of = nullptr;       // N
@@ -194,6 +196,10 @@ int launch_reference_reorder(const Geometry& g, const double* x, uint32_t* tag,
 // (kernel-level API): perm[k] = old index at slot k.
 int launch_reorder_perm(const Geometry& g, const double* x_ref, uint32_t* perm, SortTmp& t,
                         hipStream_t s);
+// dst[rank[k]] = src[k]: carries a reference-ordered array across a
+// reference reorder (rank = the tag_tmp output of launch_reference_reorder)
+int launch_rank_scatter_u64(const unsigned long long* src, unsigned long long* dst,
+                            const uint32_t* rank, int N, hipStream_t s);
 
 // --------------------------------------------------------- scatters/misc
 // dst[tag[w]*width + d] = src[w*width + d]   (working -> reference order)

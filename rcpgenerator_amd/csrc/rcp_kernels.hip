@@ -1127,11 +1127,41 @@ __global__ __launch_bounds__(kBlock) void k_place(BinArgs a)
 
 // One thread per cell: order the cell's members by their previous working
 // position, which makes the new order independent of atomic arrival order.
+// Cells of up to 16 members are sorted in registers by a bitonic network
+// (padding sorts last); larger ones by insertion in memory.
 __global__ __launch_bounds__(kBlock) void k_cell_order(BinArgs a)
 {
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= a.grid.total) return;
     const uint32_t q0 = a.cell_start[c], q1 = a.cell_start[c + 1];
+    const uint32_t n = q1 - q0;
+    if (n <= 1) return;
+    if (n <= 16) {
+        uint32_t v[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) v[k] = (uint32_t)k < n ? a.perm[q0 + k] : 0xffffffffu;
+#pragma unroll
+        for (int size = 2; size <= 16; size <<= 1) {
+#pragma unroll
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+#pragma unroll
+                for (int k = 0; k < 16; ++k) {
+                    const int l = k ^ stride;
+                    if (l > k) {
+                        const bool up = (k & size) == 0;
+                        const uint32_t lo = v[k] < v[l] ? v[k] : v[l];
+                        const uint32_t hi = v[k] < v[l] ? v[l] : v[k];
+                        v[k] = up ? lo : hi;
+                        v[l] = up ? hi : lo;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if ((uint32_t)k < n) a.perm[q0 + k] = v[k];
+        return;
+    }
     for (uint32_t p = q0 + 1; p < q1; ++p) {
         const uint32_t w = a.perm[p];
         uint32_t k = p;
@@ -1158,7 +1188,6 @@ __global__ __launch_bounds__(kBlock) void k_permute(BinArgs a)
     }
     a.D0_o[q] = a.D0[w];
     a.tag_o[q] = a.tag[w];
-    a.origin_o[q] = a.origin[w];
     if (a.fixed) a.fixed_o[q] = a.fixed[w];
 }
 
@@ -1518,6 +1547,12 @@ __global__ void k_to_ref_f64(const double* src, double* dst, const uint32_t* tag
     const size_t k = t / w, d = t % w;
     dst[(size_t)tag[k] * w + d] = src[t];
 }
+__global__ void k_rank_scatter_u64(const unsigned long long* src, unsigned long long* dst,
+                                   const uint32_t* rank, int N)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < N) dst[rank[k]] = src[k];
+}
 __global__ void k_to_ref_u64(const unsigned long long* src, unsigned long long* dst,
                              const uint32_t* tag, int N)
 {
@@ -1688,7 +1723,7 @@ bool plan_tiles(int ndim, const CellGrid& grid, long long N, int use_rk, TilePla
     return p.blocks > 0;
 }
 
-int launch_step_tile(const StepArgs& a0, const TilePlan& p, hipStream_t s)
+int launch_step_tile(const StepArgs& a0, const TilePlan& p, hipStream_t s, hipEvent_t mark)
 {
     if (a0.g.N <= 0) return 0;
     StepArgs a = a0;
@@ -1703,34 +1738,38 @@ int launch_step_tile(const StepArgs& a0, const TilePlan& p, hipStream_t s)
         case 3: k_step_tile<3, kTileBlock><<<g, kTileBlock, p.lds_bytes, s>>>(a, p.tiles); break;
         default: return 2;
     }
+    if (mark && hipEventRecord(mark, s) != hipSuccess) return 1;
     k_fold<<<1, kFoldBlock, 0, s>>>(a.partials, g, a.result);
     return check_launch();
 }
 
-int launch_step_list(const StepArgs& a, hipStream_t s)
+int launch_step_list(const StepArgs& a, hipStream_t s, hipEvent_t mark)
 {
     if (a.g.N <= 0) return 0;
     const unsigned g = grid_for(a.g.N);
     RCP_DISPATCH_ND(a.g.ndim, k_step_list, <<<g, kBlock, 0, s>>>(a, nullptr));
+    if (mark && hipEventRecord(mark, s) != hipSuccess) return 1;
     k_fold<<<1, kFoldBlock, 0, s>>>(a.partials, g, a.result);
     return check_launch();
 }
 
-int launch_step_direct(const StepArgs& a, hipStream_t s)
+int launch_step_direct(const StepArgs& a, hipStream_t s, hipEvent_t mark)
 {
     if (a.g.N <= 0) return 0;
     const unsigned g = grid_for(a.g.N, kDirectBlock);
     RCP_DISPATCH_ND(a.g.ndim, k_step_direct, <<<g, kDirectBlock, 0, s>>>(a));
+    if (mark && hipEventRecord(mark, s) != hipSuccess) return 1;
     k_fold<<<1, kFoldBlock, 0, s>>>(a.partials, g, a.result);
     return check_launch();
 }
 
 int launch_exact_step(const StepArgs& a, uint32_t* ref_count, uint32_t* ref_start,
-                      double* ref_terms, void* scan_tmp, hipStream_t s)
+                      double* ref_terms, void* scan_tmp, hipStream_t s, hipEvent_t mark)
 {
     if (a.g.N <= 0) return 0;
     const unsigned g = grid_for(a.g.N);
     RCP_DISPATCH_ND(a.g.ndim, k_step_list, <<<g, kBlock, 0, s>>>(a, ref_count));
+    if (mark && hipEventRecord(mark, s) != hipSuccess) return 1;
     k_fold<<<1, kFoldBlock, 0, s>>>(a.partials, g, a.result);
     if (launch_scan_u32(ref_count, ref_start, (size_t)a.g.N, scan_tmp, s)) return 1;
     k_exact_compact<<<g, kBlock, 0, s>>>(a, ref_start, ref_terms);
@@ -1880,6 +1919,13 @@ int launch_to_ref_f64(const double* src, double* dst, const uint32_t* tag, int N
 {
     if (N <= 0) return 0;
     k_to_ref_f64<<<grid_for((size_t)N * width), kBlock, 0, s>>>(src, dst, tag, N, width);
+    return check_launch();
+}
+int launch_rank_scatter_u64(const unsigned long long* src, unsigned long long* dst,
+                            const uint32_t* rank, int N, hipStream_t s)
+{
+    if (N <= 0) return 0;
+    k_rank_scatter_u64<<<grid_for(N), kBlock, 0, s>>>(src, dst, rank, N);
     return check_launch();
 }
 int launch_to_ref_u64(const unsigned long long* src, unsigned long long* dst, const uint32_t* tag,
