@@ -173,7 +173,7 @@ def pmc_summary(fetch_csv, write_csv):
 def read_traffic():
     """HBM bytes per launch of the fused step kernel from the committed
     rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled)."""
-    p = os.path.join(ROOT, "profiles", "k_step_2d_1M_traffic.json")
+    p = os.path.join(ROOT, "profiles", "k_step_tile_2d_1M_traffic.txt")
     try:
         with open(p) as f:
             d = json.load(f)
