@@ -257,6 +257,52 @@ void DeviceState::rebin(const Geometry& g, double reach)
     inv_tag_valid = false;
 }
 
+void DeviceState::rebin_multi(const Geometry& g, const MultiGrid& mg, double kappa)
+{
+    const size_t total = (size_t)mg.base[mg.C];
+    if (total + 1 > cell_cap) {
+        if (cell_count) {
+            release(cell_count);
+            release(cell_start);
+        }
+        cell_cap = total + 1;
+        cell_count = alloc<uint32_t>(cell_cap);
+        cell_start = alloc<uint32_t>(cell_cap);
+    }
+    ensure_scan(std::max(total, (size_t)N) + 8);
+    BinArgs b;
+    b.g = g;
+    b.grid = mg.grid[0];
+    b.grid.total = (long long)total;
+    b.kappa = kappa;
+    b.x = x;
+    b.m = m;
+    b.v = v;
+    b.D0 = D0;
+    b.tag = tag;
+    b.fixed = fixed;
+    b.x_o = x_alt;
+    b.m_o = m_alt;
+    b.v_o = v_alt;
+    b.D0_o = D0_alt;
+    b.tag_o = tag_alt;
+    b.fixed_o = fixed_alt;
+    b.cell_of = cell_of;
+    b.slot = slot;
+    b.cell_count = cell_count;
+    b.cell_start = cell_start;
+    b.perm = perm;
+    if (launch_rebin_multi(b, mg, scan_tmp, s)) throw HipFailure("size-class rebin launch");
+    std::swap(x, x_alt);
+    std::swap(m, m_alt);
+    std::swap(v, v_alt);
+    std::swap(D0, D0_alt);
+    std::swap(tag, tag_alt);
+    std::swap(fixed, fixed_alt);
+    binned = true;
+    inv_tag_valid = false;
+}
+
 void DeviceState::ensure_inv_tag()
 {
     if (inv_tag_valid) return;
@@ -382,8 +428,8 @@ Engine::Engine(const rcp_run_config& c)
     if (c.ndim < 1 || c.ndim > (size_t)kMaxDim) throw ArgError("rcp: Ndim must be in 1..8");
     if (!c.positions || !c.diameters) throw ArgError("rcp: positions/diameters missing");
     if (c.N > 0x7fffffffull) throw ArgError("rcp: N too large");
-    if (c.neighbor_mode < 0 || c.neighbor_mode > 3)
-        throw ArgError("rcp: neighbor_mode must be 0, 1, 2 or 3");
+    if (c.neighbor_mode < 0 || c.neighbor_mode > 4)
+        throw ArgError("rcp: neighbor_mode must be 0, 1, 2, 3 or 4");
     if (c.neighbor_mode >= 2 && c.exact_reductions)
         throw ArgError("rcp: exact_reductions needs neighbour lists (neighbor_mode 0 or 1)");
     if (c.neighbor_mode == 3 && c.ndim != 2 && c.ndim != 3)
@@ -511,8 +557,17 @@ Engine::Engine(const rcp_run_config& c)
     }
     kappa_D_ = 1.0;
     kappa_prev_ = 1.0;
-    build_lists();
-    const Scalars s0 = step_device(false, 0.0, 1.0, 1.0, 0, nullptr, Mode::List);
+    Mode first = Mode::List;
+    if (multi_supported()) {
+        multi_rebin(0.0);
+        first = Mode::Multi;
+    } else if (tiles_supported()) {
+        tile_rebin(0.0);
+        first = Mode::Tile;
+    } else {
+        build_lists();
+    }
+    const Scalars s0 = step_device(false, 0.0, 1.0, 1.0, 0, nullptr, first);
     finalize_scalars(s0, mu_);
 }
 
@@ -543,6 +598,36 @@ void Engine::recompute_d0_cache(const std::vector<double>& D0, const std::vector
         else D0max_free_ = std::max(D0max_free_, D0[i]);
     }
     shell0_ = reference_shell(D0);
+    // size classes: halving diameters below the largest; a frozen subset
+    // changes its size relative to the rest, so it gets one class of slack
+    const double d0max = std::max(D0max_free_, D0max_fixed_);
+    const double ratio = (D0_min_ > 0.0) ? d0max / D0_min_ : 1.0;
+    int C = 1 + (int)std::floor(std::log2(std::max(ratio, 1.0)));
+    if (has_fixed_) ++C;
+    classes_ = std::min(std::max(C, 1), kMaxClasses);
+    class_count_.assign(classes_, 0);
+    for (std::size_t i = 0; i < N_; ++i) {
+        int c = (D0[i] > 0.0) ? (int)std::floor(std::log2(d0max / D0[i])) : classes_ - 1;
+        class_count_[std::min(std::max(c, 0), classes_ - 1)] += 1;
+    }
+    // Candidates per particle, up to the common density factor: a particle
+    // of class a scans a box of width D_a + 2 D_b around it in the grid of
+    // class b (D_c = d0max 2^-c); one grid sized by the largest diameter
+    // makes that D_a + 2 d0max for every partner.  Size classes replace the
+    // single grid when they cut the candidates by more than the single-grid
+    // kernels' per-candidate advantage (LDS tiles in 2-D/3-D, rows beyond).
+    double single = 0.0, multi = 0.0;
+    for (int a = 0; a < classes_; ++a) {
+        const double Da = d0max * std::ldexp(1.0, -a);
+        single += (double)class_count_[a] * std::pow(Da + 2.0 * d0max, (double)nd_) * (double)N_;
+        for (int b = 0; b < classes_; ++b) {
+            const double Db = d0max * std::ldexp(1.0, -b);
+            multi += (double)class_count_[a] * (double)class_count_[b] *
+                     std::pow(Da + 2.0 * Db, (double)nd_);
+        }
+    }
+    const double advantage = (nd_ == 2 || nd_ == 3) ? 4.0 : 3.0;
+    polydisperse_ = classes_ > 1 && single > advantage * multi;
 }
 
 double Engine::dmax_now() const
@@ -605,10 +690,53 @@ void Engine::build_lists()
     list_entries_ += st_->nnz;
 }
 
+bool Engine::multi_supported() const
+{
+    if (opt_.exact_reductions) return false;
+    return opt_.neighbor_mode == 4 || (opt_.neighbor_mode == 0 && polydisperse_);
+}
+
 bool Engine::tiles_supported() const
 {
     if (opt_.exact_reductions || (nd_ != 2 && nd_ != 3)) return false;
-    return opt_.neighbor_mode == 0 || opt_.neighbor_mode == 3;
+    return opt_.neighbor_mode == 3 || (opt_.neighbor_mode == 0 && !polydisperse_);
+}
+
+void Engine::multi_rebin(double skin)
+{
+    // One grid per size class, cells at least as wide as the class's largest
+    // diameter plus the skin (the step kernel sizes each particle's stencil
+    // from its own reach, so the edge only sets the work per cell).
+    const double dmax = dmax_now();
+    MultiGrid& mg = mg_;
+    mg.C = classes_;
+    long long base = 0;
+    for (int c = 0; c < mg.C; ++c) {
+        const double dc = dmax * std::ldexp(1.0, -c);
+        mg.thr[c] = 0.5 * dc;
+        const long long cap =
+            std::max<long long>(64, 2 * (long long)(c < (int)class_count_.size() ? class_count_[c] : 1));
+        make_grid(geo_.box, (int)nd_, dc + skin, cap, mg.grid[c]);
+        mg.base[c] = base;
+        base += mg.grid[c].total;
+    }
+    mg.base[mg.C] = base;
+    mg.grow = 1.0;
+    // classes at least four times the smallest can touch many small
+    // particles: they take a wavefront each
+    mg.wide = std::max(0, mg.C - 2);
+    begin_timer(0);
+    st_->rebin_multi(geo_, mg, kappa_D_);
+    if (skin > 0.0)
+        hip_check(hipMemcpyAsync(st_->anchor, st_->x, N_ * nd_ * sizeof(double),
+                                 hipMemcpyDeviceToDevice, stream_), "anchor copy");
+    end_timer(0);
+    tile_skin_ = skin;
+    tile_binned_ = true;
+    kappa_b_ = kappa_D_;
+    fh_drift_ = 0.0;
+    need_build_ = true;
+    ++tile_rebins_;
 }
 
 void Engine::tile_rebin(double skin)
@@ -648,7 +776,8 @@ Engine::Scalars Engine::step_device(bool do_adam, double alpha, double inv_b1, d
     a.D0 = S.D0;
     a.fixed = S.fixed;
     a.tag = S.tag;
-    a.x_anchor = direct || (mode == Mode::Tile && tile_skin_ <= 0.0) ? nullptr : S.anchor;
+    const bool binned = mode == Mode::Tile || mode == Mode::Multi;
+    a.x_anchor = direct || (binned && tile_skin_ <= 0.0) ? nullptr : S.anchor;
     a.row_start = S.row_start;
     a.nbr = S.nbr;
     a.grid = S.grid;
@@ -668,6 +797,11 @@ Engine::Scalars Engine::step_device(bool do_adam, double alpha, double inv_b1, d
         a.dmax = dmax_now();
         a.pad = 0.5 * tile_skin_;
     }
+    if (mode == Mode::Multi) {
+        S.ensure_partials(multi_partial_rows(N_) + 1);
+        a.pad = 0.5 * tile_skin_;
+        mg_.grow = std::max(1.0, kappa_D_ / kappa_b_);
+    }
     if (opt_.exact_reductions) S.ensure_inv_tag();
     a.partials = S.partials;
     begin_timer(1);
@@ -675,6 +809,8 @@ Engine::Scalars Engine::step_device(bool do_adam, double alpha, double inv_b1, d
     if (mode == Mode::Tile) {
         if (launch_step_tile(a, tile_plan_, stream_, mark))
             throw HipFailure("tile step launch");
+    } else if (mode == Mode::Multi) {
+        if (launch_step_multi(a, mg_, stream_, mark)) throw HipFailure("size-class step launch");
     } else if (direct) {
         if (launch_step_direct(a, stream_, mark)) throw HipFailure("direct step launch");
     } else if (opt_.exact_reductions) {
@@ -957,8 +1093,9 @@ bool Engine::step_once()
     // list live long enough to pay for its build.
     const double growth_step = D0max_free_ * std::fabs(kappa_D_ - kappa_prev_);
     kappa_prev_ = kappa_D_;
-    if (tiles_supported()) {
-        mode_ = Mode::Tile;
+    const bool multi = multi_supported();
+    if (multi || tiles_supported()) {
+        mode_ = multi ? Mode::Multi : Mode::Tile;
         // the binning holds while no particle can have crossed half the skin
         const double growth = std::max(0.0, D0max_free_ * (kappa_D_ - kappa_b_));
         const double reach = 2.0 * std::sqrt(maxdisp2_) + growth + 2.0 * fh_drift_;
@@ -967,7 +1104,9 @@ bool Engine::step_once()
             // a skin pays off once a binning lives two steps or more
             const double per_step = 2.0 * std::sqrt(maxstep2_) + growth_step + 2.0 * fh_step_;
             const double shell = shell0_ * kappa_D_;
-            tile_rebin(per_step * 2.0 < shell ? shell : 0.0);
+            const double skin = per_step * 2.0 < shell ? shell : 0.0;
+            if (multi) multi_rebin(skin);
+            else tile_rebin(skin);
             tile_invalid_ = false;
         }
     } else if (opt_.neighbor_mode == 2) {
@@ -995,7 +1134,7 @@ bool Engine::step_once()
         }
     }
     const bool direct = (mode_ == Mode::Direct);
-    if (mode_ == Mode::Tile) {
+    if (mode_ == Mode::Tile || mode_ == Mode::Multi) {
         ++tile_steps_;
     } else if (direct) {
         begin_timer(0);

@@ -60,6 +60,8 @@ public:
     void upload(const double* x, const double* D0, const int8_t* fixed);
     // re-bin into cell order for cells at least `reach` wide
     void rebin(const Geometry& g, double reach);
+    // re-bin by size class into class-major cell order
+    void rebin_multi(const Geometry& g, const MultiGrid& mg, double kappa);
     // rows of the current binning: rc = (Di + Dj)*0.5 + t_shell
     void build_rows(const Geometry& g, double kappa, double t_shell, int half, int sortwalk_rc);
     // reference reorder of the tags
@@ -165,7 +167,7 @@ public:
 private:
     struct Scalars { double v[S_NUM]; };
     enum class Osc { Inactive, Active, Final, Done };
-    enum class Mode { List, Direct, Tile };
+    enum class Mode { List, Direct, Tile, Multi };
 
     void recompute_d0_cache(const std::vector<double>& D0, const std::vector<int8_t>& fixed);
     double dmax_now() const;
@@ -173,7 +175,9 @@ private:
     Scalars step_device(bool do_adam, double alpha, double inv_b1, double inv_b2, int use_rk,
                         const double* rk, Mode mode);
     void tile_rebin(double skin);
+    void multi_rebin(double skin);
     bool tiles_supported() const;
+    bool multi_supported() const;
     void finalize_scalars(const Scalars& s, double mu_eff);
     bool schedule_top(std::size_t step);
     void schedule_alpha(std::size_t step);
@@ -232,6 +236,10 @@ private:
     bool tile_binned_ = false, tile_invalid_ = false;
     uint64_t tile_rebins_ = 0, tile_steps_ = 0;
     TilePlan tile_plan_;
+    MultiGrid mg_;
+    int classes_ = 1;
+    bool polydisperse_ = false;
+    std::vector<long long> class_count_;
     uint64_t list_builds_ = 0, list_entries_ = 0, reorders_ = 0, direct_steps_total_ = 0,
              overflow_steps_ = 0;
 

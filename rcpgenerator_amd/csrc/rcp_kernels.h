@@ -52,6 +52,22 @@ struct CellGrid {
     long long total = 1;
 };
 
+// Size classes for wide diameter distributions: class c holds the particles
+// with thr[c] < D (the first such c; the last class takes the rest), thr[c]
+// = dmax_bin * 2^-(c+1) at binning time, and has its own cell grid.  Class c
+// occupies cells base[c] .. base[c+1]-1 of one cell table.  At a later step
+// every diameter in class c is at most 2 thr[c] grow, grow = max(1, kappa /
+// kappa at binning).
+constexpr int kMaxClasses = 8;
+struct MultiGrid {
+    int C = 1;
+    int wide = 0;        // classes 0..wide-1 (the largest) take one wavefront per particle
+    double grow = 1.0;
+    double thr[kMaxClasses] = {};
+    CellGrid grid[kMaxClasses];
+    long long base[kMaxClasses + 1] = {};
+};
+
 // Fused step over the working order.
 struct StepArgs {
     Geometry g;
@@ -124,7 +140,8 @@ int launch_exact_step(const StepArgs& a, uint32_t* ref_count, uint32_t* ref_star
 // maintained here (launch_inv_tag rebuilds it when needed).
 struct BinArgs {
     Geometry g;
-    CellGrid grid;
+    CellGrid grid;                     // single grid (grid.total = cells in the table)
+    double kappa = 1.0;                // size classes: D = fixed ? D0 : D0 * kappa
     // old working order
     const double* x = nullptr;
     const double* m = nullptr;
@@ -147,6 +164,14 @@ struct BinArgs {
     uint32_t* perm = nullptr;          // N: old position of new position q
 };
 int launch_rebin(BinArgs& a, void* scan_tmp, hipStream_t s);
+// Re-binning by size class (a.grid.total = mg.base[mg.C]).
+int launch_rebin_multi(BinArgs& a, const MultiGrid& mg, void* scan_tmp, hipStream_t s);
+// Step over size-class grids: each particle visits, in every class grid,
+// the cells within (Di + max D of the class)/2 + a.pad of itself.
+int launch_step_multi(const StepArgs& a, const MultiGrid& mg, hipStream_t s,
+                      hipEvent_t mark = nullptr);
+// partial rows a size-class step of n particles writes
+size_t multi_partial_rows(size_t n);
 // inv_tag[tag[w]] = w
 int launch_inv_tag(const uint32_t* tag, uint32_t* inv_tag, int N, hipStream_t s);
 

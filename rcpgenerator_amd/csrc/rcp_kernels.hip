@@ -940,6 +940,320 @@ __global__ __launch_bounds__(BLOCK) void k_step_tile(StepArgs a, long long ntile
     finish_grid<BLOCK>(a, blk);
 }
 
+// ------------------------------------------------------ size-class step
+//
+// Thread q visits, in each class grid b, the cells overlapping
+// [x - R, x + R] per axis, R = (Di + 2 thr[b] grow)/2 + pad (contacts are
+// closer than (Di + Dj)/2 and partners drifted less than pad from where they
+// were binned); the stencil wraps periodically and covers an axis once at
+// most.  Contacts are buffered as working positions in LDS, sorted by tag
+// and accumulated; beyond the buffer, repeated scans take them in tag order.
+template <int ND, class Fn>
+__device__ __forceinline__ void for_each_class_candidate(const StepArgs& a, const MultiGrid& mg,
+                                                         const double* x, double D, Fn&& fn)
+{
+    for (int b = 0; b < mg.C; ++b) {
+        const CellGrid& gr = mg.grid[b];
+        const double R = 0.5 * (D + 2.0 * mg.thr[b] * mg.grow) + a.pad;
+        int lo[ND], cnt[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            const int nc = gr.nc[d];
+            if (nc < 3) {
+                lo[d] = 0;
+                cnt[d] = 1;
+                continue;
+            }
+            const double s = gr.cell_scale[d];
+            const long long l = (long long)floor((x[d] - R) * s - 1e-9);
+            const long long h = (long long)floor((x[d] + R) * s + 1e-9);
+            if (h - l + 1 >= nc) {
+                lo[d] = 0;
+                cnt[d] = nc;
+            } else {
+                long long m = l % nc;
+                if (m < 0) m += nc;
+                lo[d] = (int)m;
+                cnt[d] = (int)(h - l + 1);
+            }
+        }
+        int ncell = 1;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) ncell *= cnt[d];
+        for (int s = 0; s < ncell; ++s) {
+            int t = s;
+            long long cc = 0, stride = 1;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                int k = lo[d] + t % cnt[d];
+                t /= cnt[d];
+                if (k >= gr.nc[d]) k -= gr.nc[d];
+                cc += (long long)k * stride;
+                stride *= gr.nc[d];
+            }
+            cc += mg.base[b];
+            const uint32_t q0 = a.cell_start[cc], q1 = a.cell_start[cc + 1];
+            for (uint32_t qj = q0; qj < q1; ++qj) fn(qj);
+        }
+    }
+}
+
+constexpr int kMultiBlock = 128;
+constexpr int kMultiCap = 24;
+
+template <int ND>
+__global__ __launch_bounds__(kMultiBlock) void k_step_multi(StepArgs a, MultiGrid mg)
+{
+    __shared__ uint32_t buf[kMultiCap][kMultiBlock];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tid = threadIdx.x;
+    double acc[S_NUM];
+#pragma unroll
+    for (int k = 0; k < S_NUM; ++k) acc[k] = 0.0;
+    // the largest classes lead the working order; k_step_wide takes them
+    const uint32_t n_wide = mg.wide > 0 ? a.cell_start[mg.base[mg.wide]] : 0u;
+    if (q < a.g.N && (uint32_t)q >= n_wide) {
+        Own<ND> o;
+        load_own<ND>(a, q, o);
+        double F[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) F[d] = 0.0;
+        int n = 0;
+        for_each_class_candidate<ND>(a, mg, o.x, o.D, [&](uint32_t qj) {
+            if (qj == (uint32_t)q) return;
+            if (!overlaps<ND>(a, o, qj)) return;
+            if (n < kMultiCap) buf[n][tid] = qj;
+            ++n;
+        });
+        if (n <= kMultiCap) {
+            for (int p = 1; p < n; ++p) {
+                const uint32_t jk = buf[p][tid];
+                const uint32_t tk = a.tag[jk];
+                int k = p;
+                while (k > 0 && a.tag[buf[k - 1][tid]] > tk) {
+                    buf[k][tid] = buf[k - 1][tid];
+                    --k;
+                }
+                buf[k][tid] = jk;
+            }
+            for (int p = 0; p < n; ++p) {
+                const uint32_t j = buf[p][tid];
+                contact<ND>(a, o, j, a.tag[j], F, acc, nullptr);
+            }
+        } else {
+            acc[S_OVERFLOW] += 1.0;
+            long long last = -1;
+            while (true) {
+                unsigned long long best = ~0ull;
+                for_each_class_candidate<ND>(a, mg, o.x, o.D, [&](uint32_t qj) {
+                    if (qj == (uint32_t)q) return;
+                    const uint32_t tj = a.tag[qj];
+                    if ((long long)tj <= last) return;
+                    const unsigned long long key = ((unsigned long long)tj << 32) | qj;
+                    if (key >= best) return;
+                    if (overlaps<ND>(a, o, qj)) best = key;
+                });
+                if (best == ~0ull) break;
+                contact<ND>(a, o, (uint32_t)(best & 0xffffffffull), (uint32_t)(best >> 32), F, acc,
+                            nullptr);
+                last = (long long)(best >> 32);
+            }
+        }
+        particle_tail<ND>(a, q, o, F, acc);
+    }
+    double blk[S_NUM];
+    block_reduce<kMultiBlock>(acc, blk);
+    finish_grid<kMultiBlock>(a, blk);
+}
+
+// One wavefront per particle of the largest classes (a big particle can
+// touch hundreds of small ones).  The lanes split the stencil cells and
+// append contacts as (tag << 32 | position) keys to LDS; a wave-wide bitonic
+// sort orders them; lanes then compute the pair terms of 64 contacts at a
+// time in parallel and lane 0 adds them in tag order, so the sums round as
+// in the per-thread kernels.  Persistent: wave w of the grid takes particles
+// w, w + waves, ...
+// Orders the LDS accesses of one wavefront (the waves of a workgroup work
+// on their own LDS regions and never wait for each other).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int kWideWaves = 4;                  // wavefronts per workgroup
+constexpr int kWideGrid = 1024;                // persistent workgroups
+constexpr int kWideCap = 1024;                 // contacts buffered per particle
+constexpr int kWideBlock = kWideWaves * kWave;
+
+template <int ND>
+__global__ __launch_bounds__(kWideBlock) void k_step_wide(StepArgs a, MultiGrid mg)
+{
+    __shared__ unsigned long long keys[kWideWaves][kWideCap];
+    __shared__ double prod[kWideWaves][ND][kWave];
+    __shared__ double terms[kWideWaves][4][kWave];
+    __shared__ int flag[kWideWaves][kWave];
+    __shared__ int cnt[kWideWaves];
+    const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+    double acc[S_NUM];
+#pragma unroll
+    for (int k = 0; k < S_NUM; ++k) acc[k] = 0.0;
+    const uint32_t n_wide = a.cell_start[mg.base[mg.wide]];
+    const unsigned waves = gridDim.x * kWideWaves;
+    for (uint32_t q = blockIdx.x * kWideWaves + wv; q < n_wide; q += waves) {
+        Own<ND> o;
+        load_own<ND>(a, (int)q, o);
+        if (lane == 0) cnt[wv] = 0;
+        wave_sync();
+        // ---- collect: lanes take the stencil cells of every class in turn
+        for (int b = 0; b < mg.C; ++b) {
+            const CellGrid& gr = mg.grid[b];
+            const double R = 0.5 * (o.D + 2.0 * mg.thr[b] * mg.grow) + a.pad;
+            int lo[ND], ct[ND];
+            int ncell = 1;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                const int nc = gr.nc[d];
+                lo[d] = 0;
+                ct[d] = nc < 3 ? 1 : nc;
+                if (nc >= 3) {
+                    const double s = gr.cell_scale[d];
+                    const long long l = (long long)floor((o.x[d] - R) * s - 1e-9);
+                    const long long h = (long long)floor((o.x[d] + R) * s + 1e-9);
+                    if (h - l + 1 < nc) {
+                        long long m = l % nc;
+                        if (m < 0) m += nc;
+                        lo[d] = (int)m;
+                        ct[d] = (int)(h - l + 1);
+                    }
+                }
+                ncell *= ct[d];
+            }
+            for (int s = lane; s < ncell; s += kWave) {
+                int t = s;
+                long long cc = 0, stride = 1;
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    int k = lo[d] + t % ct[d];
+                    t /= ct[d];
+                    if (k >= gr.nc[d]) k -= gr.nc[d];
+                    cc += (long long)k * stride;
+                    stride *= gr.nc[d];
+                }
+                cc += mg.base[b];
+                const uint32_t q0 = a.cell_start[cc], q1 = a.cell_start[cc + 1];
+                for (uint32_t qj = q0; qj < q1; ++qj) {
+                    if (qj == q || !overlaps<ND>(a, o, qj)) continue;
+                    const int slot = atomicAdd(&cnt[wv], 1);
+                    if (slot < kWideCap)
+                        keys[wv][slot] = ((unsigned long long)a.tag[qj] << 32) | qj;
+                }
+            }
+        }
+        wave_sync();
+        const int n = cnt[wv];
+        double F[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) F[d] = 0.0;
+        if (n <= kWideCap) {
+            // ---- bitonic sort of the keys (padding sorts last)
+            int P = 1;
+            while (P < n) P <<= 1;
+            for (int i = n + lane; i < P; i += kWave) keys[wv][i] = ~0ull;
+            wave_sync();
+            for (int size = 2; size <= P; size <<= 1) {
+                for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                    for (int i = lane; i < P; i += kWave) {
+                        const int l = i ^ stride;
+                        if (l > i) {
+                            const unsigned long long ki = keys[wv][i], kl = keys[wv][l];
+                            const bool up = (i & size) == 0;
+                            if ((ki > kl) == up) {
+                                keys[wv][i] = kl;
+                                keys[wv][l] = ki;
+                            }
+                        }
+                    }
+                    wave_sync();
+                }
+            }
+            // ---- pair terms 64 at a time, summed in order by lane 0
+            for (int c0 = 0; c0 < n; c0 += kWave) {
+                const int p = c0 + lane;
+                if (p < n) {
+                    const unsigned long long key = keys[wv][p];
+                    const uint32_t j = (uint32_t)(key & 0xffffffffull);
+                    const uint32_t tj = (uint32_t)(key >> 32);
+                    double xj[ND], Dj, rkj;
+                    load_partner<ND>(a, j, xj, Dj, rkj);
+                    double Fp[ND];
+#pragma unroll
+                    for (int d = 0; d < ND; ++d) Fp[d] = 0.0;
+                    double tacc[S_NUM];
+#pragma unroll
+                    for (int k = 0; k < S_NUM; ++k) tacc[k] = 0.0;
+                    double rec[4] = {0.0, 0.0, 0.0, 0.0};
+                    pair_contact<ND>(a, o, xj, Dj, rkj, tj, Fp, tacc, rec);
+                    // Fp = 0 + fm*dl: the product the sequential sum adds
+#pragma unroll
+                    for (int d = 0; d < ND; ++d) prod[wv][d][lane] = Fp[d];
+                    flag[wv][lane] = tj > o.tag ? 1 : 0;
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) terms[wv][k][lane] = rec[k];
+                }
+                wave_sync();
+                if (lane == 0) {
+                    const int m = (n - c0 < kWave) ? n - c0 : kWave;
+                    for (int u = 0; u < m; ++u) {
+#pragma unroll
+                        for (int d = 0; d < ND; ++d) F[d] += prod[wv][d][u];
+                        if (flag[wv][u]) {
+                            const double ov = terms[wv][0][u];
+                            acc[S_U] += ov;
+                            acc[S_CNT] += 1.0;
+                            acc[S_LC] += terms[wv][1][u];
+                            acc[S_MAXOV] = (ov > acc[S_MAXOV]) ? ov : acc[S_MAXOV];
+                            acc[S_DK] = acc[S_DK] + terms[wv][2][u];
+                            acc[S_FM] = acc[S_FM] + terms[wv][3][u];
+                        }
+                    }
+                }
+                wave_sync();
+            }
+        } else if (lane == 0) {
+            // more contacts than the buffer: repeated scans in tag order
+            acc[S_OVERFLOW] += 1.0;
+            long long last = -1;
+            while (true) {
+                unsigned long long best = ~0ull;
+                for_each_class_candidate<ND>(a, mg, o.x, o.D, [&](uint32_t qj) {
+                    if (qj == q) return;
+                    const uint32_t tj = a.tag[qj];
+                    if ((long long)tj <= last) return;
+                    const unsigned long long key = ((unsigned long long)tj << 32) | qj;
+                    if (key >= best) return;
+                    if (overlaps<ND>(a, o, qj)) best = key;
+                });
+                if (best == ~0ull) break;
+                contact<ND>(a, o, (uint32_t)(best & 0xffffffffull), (uint32_t)(best >> 32), F, acc,
+                            nullptr);
+                last = (long long)(best >> 32);
+            }
+        }
+        if (lane == 0) particle_tail<ND>(a, (int)q, o, F, acc);
+        wave_sync();
+    }
+    double blk[S_NUM];
+    block_reduce<kWideBlock>(acc, blk);
+    if (threadIdx.x == 0) {
+        // rows after the per-thread kernel's rows
+        const size_t row = (size_t)a.tile_len + blockIdx.x;
+#pragma unroll
+        for (int k = 0; k < S_NUM; ++k) a.partials[row * S_NUM + k] = blk[k];
+    }
+}
+
 // -------------------------------------------------------- exact reduction
 //
 // The recorded pair terms are first compacted into reference order (rows by
@@ -1116,6 +1430,28 @@ __global__ __launch_bounds__(kBlock) void k_bin(BinArgs a)
     }
     a.cell_of[w] = (uint32_t)c;
     a.slot[w] = atomicAdd(&a.cell_count[c], 1u);
+}
+
+template <int ND>
+__global__ __launch_bounds__(kBlock) void k_bin_multi(BinArgs a, MultiGrid mg)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= a.g.N) return;
+    const bool f = a.fixed && a.fixed[w];
+    const double D = f ? a.D0[w] : a.D0[w] * a.kappa;
+    int c = 0;
+    while (c < mg.C - 1 && !(D > mg.thr[c])) ++c;
+    const CellGrid& gr = mg.grid[c];
+    long long cell = 0, stride = 1;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        const int k = cell_coord(a.x[(size_t)w * ND + d], a.g.box[d], gr.cell_scale[d], gr.nc[d]);
+        cell += (long long)k * stride;
+        stride *= gr.nc[d];
+    }
+    cell += mg.base[c];
+    a.cell_of[w] = (uint32_t)cell;
+    a.slot[w] = atomicAdd(&a.cell_count[cell], 1u);
 }
 
 __global__ __launch_bounds__(kBlock) void k_place(BinArgs a)
@@ -1753,6 +2089,28 @@ int launch_step_list(const StepArgs& a, hipStream_t s, hipEvent_t mark)
     return check_launch();
 }
 
+size_t multi_partial_rows(size_t n)
+{
+    return grid_for(n, kMultiBlock) + (size_t)kWideGrid;
+}
+
+int launch_step_multi(const StepArgs& a0, const MultiGrid& mg, hipStream_t s, hipEvent_t mark)
+{
+    if (a0.g.N <= 0) return 0;
+    StepArgs a = a0;
+    const unsigned g = grid_for(a.g.N, kMultiBlock);
+    RCP_DISPATCH_ND(a.g.ndim, k_step_multi, <<<g, kMultiBlock, 0, s>>>(a, mg));
+    unsigned rows = g;
+    if (mg.wide > 0) {
+        a.tile_len = (int)g;   // first partial row of k_step_wide
+        RCP_DISPATCH_ND(a.g.ndim, k_step_wide, <<<(unsigned)kWideGrid, kWideBlock, 0, s>>>(a, mg));
+        rows += (unsigned)kWideGrid;
+    }
+    if (mark && hipEventRecord(mark, s) != hipSuccess) return 1;
+    k_fold<<<1, kFoldBlock, 0, s>>>(a.partials, rows, a.result);
+    return check_launch();
+}
+
 int launch_step_direct(const StepArgs& a, hipStream_t s, hipEvent_t mark)
 {
     if (a.g.N <= 0) return 0;
@@ -1783,6 +2141,19 @@ int launch_rebin(BinArgs& a, void* scan_tmp, hipStream_t s)
     const size_t total = (size_t)a.grid.total;
     k_fill_u32<<<grid_for(total + 1), kBlock, 0, s>>>(a.cell_count, 0u, total + 1);
     RCP_DISPATCH_ND(a.g.ndim, k_bin, <<<grid_for(N), kBlock, 0, s>>>(a));
+    if (launch_scan_u32(a.cell_count, a.cell_start, total, scan_tmp, s)) return 1;
+    k_place<<<grid_for(N), kBlock, 0, s>>>(a);
+    k_cell_order<<<grid_for(total), kBlock, 0, s>>>(a);
+    RCP_DISPATCH_ND(a.g.ndim, k_permute, <<<grid_for(N), kBlock, 0, s>>>(a));
+    return check_launch();
+}
+
+int launch_rebin_multi(BinArgs& a, const MultiGrid& mg, void* scan_tmp, hipStream_t s)
+{
+    const int N = a.g.N;
+    const size_t total = (size_t)a.grid.total;
+    k_fill_u32<<<grid_for(total + 1), kBlock, 0, s>>>(a.cell_count, 0u, total + 1);
+    RCP_DISPATCH_ND(a.g.ndim, k_bin_multi, <<<grid_for(N), kBlock, 0, s>>>(a, mg));
     if (launch_scan_u32(a.cell_count, a.cell_start, total, scan_tmp, s)) return 1;
     k_place<<<grid_for(N), kBlock, 0, s>>>(a);
     k_cell_order<<<grid_for(total), kBlock, 0, s>>>(a);

@@ -339,6 +339,49 @@ def test_neighbour_modes_agree(gpu, nd, phi, cluster):
             assert sc["overflow_steps"] > 0
 
 
+def powerlaw_config(rng, N, nd, phi, ratio=10.0):
+    """Diameters with density ~ d^-3 on [1/ratio, 1] (10:1 polydispersity)."""
+    dmin, dmax = 1.0 / ratio, 1.0
+    u = rng.random(N)
+    D = (dmin ** -2 - u * (dmin ** -2 - dmax ** -2)) ** -0.5
+    unit = math.pi ** (nd / 2) / math.gamma(nd / 2 + 1)
+    D *= (phi / (unit * np.sum((D / 2) ** nd))) ** (1.0 / nd)
+    return rng.random((N, nd)), D, np.ones(nd)
+
+
+@pytest.mark.parametrize("nd,phi,modes", [(2, 0.9, (1, 2, 3, 4)), (3, 0.6, (1, 2, 3, 4)),
+                                          (4, 0.4, (1, 2, 4))])
+def test_size_classes_agree(gpu, nd, phi, modes):
+    """10:1 power-law diameters: the size-class kernel (neighbor_mode 4)
+    gives the same trajectory, bit for bit, as lists, cell scan and tiles."""
+    rng = np.random.default_rng(700 + nd)
+    x, D, box = powerlaw_config(rng, 8000 if nd < 4 else 3000, nd, phi)
+    out = {}
+    for mode in modes:
+        e = gpu.Engine(x, D, box=box, max_steps=30, fix_diameter=True, neighbor_mode=mode)
+        e.run(0)
+        out[mode] = (e.result(), e.stats())
+    ref = out[1][0]["positions"]
+    for mode in modes[1:]:
+        assert same(ref, out[mode][0]["positions"]), mode
+    assert out[4][1]["tile_steps"] == 30
+
+
+def test_auto_mode_size_classes_3d(gpu):
+    import rcpgenerator_amd as rcp
+    init = rcp.initialize_particles({"phi": 0.05, "N": 20000, "Ndim": 3, "box": [1.0] * 3,
+                                     "walls": [0, 0, 1], "seed": 123,
+                                     "dist": {"type": "powerlaw", "d_min": 0.1, "d_max": 1.0,
+                                              "exponent": -3.0}})
+    e = gpu.Engine(init["positions"], init["diameters"], box=init["box"], walls=init["walls"],
+                   max_steps=3000)
+    e.run(0)
+    r, st = e.result(), e.stats()
+    assert st["tile_steps"] == r["steps"] == 3000 and st["list_builds"] == 0
+    assert np.all(np.isfinite(r["positions"]))
+    assert np.all(np.isfinite(r["phi_history"]))
+
+
 def test_auto_mode_tiles_2d(gpu):
     import rcpgenerator_amd as rcp
     init = rcp.initialize_particles({"phi": 0.1, "N": 50000, "Ndim": 2, "box": [1.0, 1.0],
