@@ -170,6 +170,131 @@ def pmc_summary(fetch_csv, write_csv):
     return out
 
 
+def batch_workload(P: int, seed0: int = 0):
+    """Config-4 generator: packing i (seed i) draws N ~ U{500..20000}, Ndim
+    from {2, 3, 4}, a diameter family (mono; bidisperse 1:1.4; log-normal
+    with cv ~ U[0.1, 0.3]; power law with ratio ~ U[2, 10], exponent -2.5),
+    periodic or hard wall per dimension and phi ~ U[0.05, 0.15]."""
+    import random
+    cases = []
+    for i in range(seed0, seed0 + P):
+        r = random.Random(i)
+        N = r.randint(500, 20000)
+        nd = r.choice((2, 3, 4))
+        fam = r.choice(("mono", "bidisperse", "lognormal", "powerlaw"))
+        if fam == "mono":
+            dist = {"type": "mono", "d": 1.0}
+        elif fam == "bidisperse":
+            dist = {"type": "bidisperse", "d1": 1.0, "d2": 1.4, "p": 0.5}
+        elif fam == "lognormal":
+            cv = r.uniform(0.1, 0.3)
+            s2 = math.log(1.0 + cv * cv)
+            dist = {"type": "lognormal", "mu": -0.5 * s2, "sigma": math.sqrt(s2)}
+        else:
+            ratio = r.uniform(2.0, 10.0)
+            dist = {"type": "powerlaw", "d_min": 1.0 / ratio, "d_max": 1.0, "exponent": -2.5}
+        walls = [r.choice((0, 1)) for _ in range(nd)]
+        phi = r.uniform(0.05, 0.15)
+        cases.append({"phi": phi, "N": N, "Ndim": nd, "box": [1.0] * nd, "walls": walls,
+                      "seed": i, "dist": dist})
+    return cases
+
+
+def bench_batch(args) -> int:
+    """Batched packings: P packings of the config-4 generator run to jamming
+    through rcp_run_batch (C ABI, summaries only: no result dicts); value =
+    packings per hour of wall time."""
+    import rcpgenerator_amd as rcp  # noqa: F401  (loads the library or fails)
+    from concurrent.futures import ThreadPoolExecutor
+    from rcpgenerator_amd import capi
+    t0 = time.perf_counter()
+    cases = batch_workload(args.packings)
+    with ThreadPoolExecutor(host_cores()) as ex:
+        inits = list(ex.map(capi.initialize, cases))
+    log(f"[bench] initializer {time.perf_counter() - t0:.1f}s for {len(cases)} packings, "
+        f"{sum(c['N'] for c in cases)} particles")
+    runs = [(x, D, b, w, None) for (x, D, b, w) in inits]
+    t0 = time.perf_counter()
+    res, st = capi.run_batch(runs, threads=args.threads, pool=args.pool)
+    dt = time.perf_counter() - t0
+    bad = [r for r in res if r is None or r["rc"] != 0]
+    if bad:
+        log(f"[bench] {len(bad)} packings failed: {bad[:3]}")
+        return 1
+    steps = [r["steps"] for r in res]
+    def cap(c):
+        # default step cap (rcp_generator.cpp:4713-4737, speed preset "quick")
+        rung1 = max(1500.0, 2.0 * 4000.0 + 9.0 * math.sqrt(c["N"] * c["Ndim"]) - 2000.0)
+        return max(60000, int(rung1 * 5))
+    # jammed: stopped by the convergence gate before the cap, overlaps small
+    jammed = sum(1 for c, r in zip(cases, res)
+                 if r["steps"] < cap(c) and r["max_min_dist"] < 5e-4)
+    upd = sum(c["N"] * s for c, s in zip(cases, steps))
+    value = len(res) / (dt / 3600.0)
+    engine = "engine pool" if args.pool else "batched engine"
+    line = {"metric": f"jammed packings/hour ({engine}, plain method)", "value": value,
+            "unit": "packings/hour", "n_gpus": 1, "packings": len(res), "jammed": jammed,
+            "wall_s": dt, "particle_updates_per_s": upd / dt, "steps_total": sum(steps),
+            "steps_max": max(steps), "threads": args.threads, "higher_is_better": True,
+            "rounds": st["rounds"], "device_s": st["device_ms"] / 1e3,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "config-4 generator (seeds 0..P-1): N ~ U{500..20000}, "
+                                   "Ndim in {2,3,4}, mono/bidisperse/log-normal/power-law, "
+                                   "walls per dim, phi ~ U[0.05,0.15]; run to the "
+                                   "reference's convergence gate or step cap"}}
+    if not args.no_cpu:
+        try:
+            inputs = [{"positions": x.tolist(), "diameters": D.tolist()}
+                      for (x, D, b, w) in inits]
+            configs = [{"box": b.tolist(), "walls": [int(v) for v in w]}
+                       for (x, D, b, w) in inits]
+            line["cpu_baseline"] = cpu_baseline_batch(inputs, configs, steps, args.cpu_budget)
+            if line["cpu_baseline"]:
+                line["gpu_over_cpu"] = round(value / line["cpu_baseline"]["value"], 1)
+        except Exception as e:  # the GPU figure stands without the baseline
+            log(f"[bench] cpu baseline failed: {e}")
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def _ref_steps_time(payload):
+    """Worker: seconds per step of the reference engine (one thread) on one
+    packing, from runs of max_steps 1 and 1 + K."""
+    inp, cfg, K, native = payload
+    from oracle import oracle as O
+    mod = O.ref_module(native=native)
+    mod.set_num_threads(1)
+    t0 = time.perf_counter()
+    mod.run_packing_observed(inp, cfg, options={"max_steps": 1})
+    t1 = time.perf_counter()
+    mod.run_packing_observed(inp, cfg, options={"max_steps": 1 + K})
+    t2 = time.perf_counter()
+    return max(((t2 - t1) - (t1 - t0)) / K, 1e-9)
+
+
+def cpu_baseline_batch(inputs, configs, steps, budget_s):
+    """The reference's batch form (rcptools: a process pool of one-thread
+    engines on all host cores).  Bounded sample: every packing runs K steps
+    on the reference; its full run is costed at the step count this engine
+    needed for it; packings/hour = 3600 * workers / sum(cost)."""
+    from oracle import oracle as O
+    if not O.have_ref():
+        return None
+    import multiprocessing as mp
+    native = has_avx512() and os.path.exists(
+        os.path.join(O.REF_DIR, "native", "_rcpgenerator" + O.EXT_SUFFIX))
+    workers = host_cores()
+    K = 20
+    with mp.get_context("spawn").Pool(workers) as pool:
+        per = pool.map(_ref_steps_time, [(i, c, K, native) for i, c in zip(inputs, configs)])
+    cost = sum(s * t for s, t in zip(steps, per))
+    return {"value": 3600.0 * workers / cost, "unit": "packings/hour", "cores": workers,
+            "kind": "reference",
+            "sample": f"each packing run {K} steps on the reference ({'AVX-512' if native else 'portable'} "
+                      f"build, 1 thread per process, {workers} processes); full runs costed at "
+                      f"this engine's step counts"}
+
+
 def read_traffic():
     """HBM bytes per launch of the fused step kernel from the committed
     rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled)."""
@@ -192,6 +317,11 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
+    ap.add_argument("--workload", choices=("single", "batch"), default="single")
+    ap.add_argument("--packings", type=int, default=64)
+    ap.add_argument("--threads", type=int, default=8)
+    ap.add_argument("--pool", action="store_true",
+                    help="batch workload on the engine pool instead of the batched engine")
     args = ap.parse_args()
     if args.pmc_summary:
         print(json.dumps(pmc_summary(*args.pmc_summary), indent=1))
@@ -201,6 +331,8 @@ def main() -> int:
               "GPU only; run with --gpus 1", file=sys.stderr)
         return 2
 
+    if args.workload == "batch":
+        return bench_batch(args)
     if args.copy_calibration:
         copy_calibration()
     import rcpgenerator_amd as rcp

@@ -64,10 +64,13 @@ typedef struct rcp_run_config {
     int has_initial_phi_target; double initial_phi_target;
     int exact_reductions;         /* 1 = reference one-thread summation order */
     int time_kernels;             /* 1 = HIP-event timing of every fused-step launch */
-    int neighbor_mode;            /* 0 = automatic (lists when they live long enough,
-                                     per-step cell scan otherwise), 1 = lists only,
-                                     2 = per-step cell scan only (not with
-                                     exact_reductions) */
+    int neighbor_mode;            /* 0 = automatic, 1 = lists only, 2 = per-step cell
+                                     scan only, 3 = tiled cells (ndim 2, 3), 4 = size-class
+                                     cells (2-4 not with exact_reductions) */
+    size_t rung1_deadline;        /* the reference's RCP_RUNG1_DEADLINE hook
+                                     (rcp_generator.cpp:5864-5868, 4727-4730) as a per-run
+                                     value: 100 < v < 1e7 replaces the preset deadline of
+                                     rung 1 (and the step cap's rung-1 term); 0 = preset */
 } rcp_run_config;
 
 /* Fills the reference defaults (reorder_interval 2000, everything else 0). */
@@ -159,6 +162,39 @@ int rcp_run_packing_observed(const rcp_run_config* cfg, size_t progress_interval
                              int capture_positions, size_t trajectory_interval,
                              rcp_observer_fn fn, void* user, rcp_engine** out);
 
+/* Completion callback of rcp_run_batch, called once per packing when it
+ * has finished: rc is its status and e its engine (NULL when rc != RCP_OK;
+ * then rcp_last_error() on the calling thread has the message).  The engine
+ * is destroyed when the callback returns, so read results inside it.  Calls
+ * for different packings may run concurrently (engine-pool workers). */
+typedef void (*rcp_batch_sink)(void* user, size_t index, int rc, rcp_engine* e);
+
+enum {
+    RCP_BATCH_POOL = 1    /* every packing on the engine pool (no batched kernels) */
+};
+
+/* Batched entry (the batch counterpart of rcp_run_packing_observed; the
+ * reference runs such batches as a process pool of single-threaded engines,
+ * rcptools/jobs.py:392-401, 531-557).  Runs `count` independent packings
+ * to completion on the current device.
+ *
+ * Packings with the fast reductions, neighbor_mode 0 or 4, no frozen
+ * diameters, fix_height or target_phi run in the batched engine: one set of
+ * device arrays per dimension, every step one launch per kernel for all of
+ * them (size-class cells), each packing with its own schedule controller;
+ * each follows its single run with neighbor_mode 4 bit for bit.  The rest
+ * (or all, with RCP_BATCH_POOL in flags) run on `threads` host workers
+ * (0: eight), each driving its own engines, largest first (N * ndim).
+ * Returns RCP_OK when every packing finished, else the first failing
+ * status. */
+int rcp_run_batch(const rcp_run_config* cfgs, size_t count, int threads, int flags,
+                  rcp_batch_sink sink, void* user);
+
+/* Counters of the last rcp_run_batch on this thread: launch rounds of the
+ * batched engine, packing-steps they advanced, and the device time of
+ * their re-binning and step launches (ms). */
+int rcp_batch_last_stats(uint64_t* rounds, uint64_t* packing_steps, double* device_ms);
+
 /* ------------------------------------------------------- kernel level
  * Host-array wrappers around single device kernels, for parity tests and
  * for callers that used the reference's exported kernels (rcp_generator.hpp
@@ -230,6 +266,30 @@ void rcp_init_config_init(rcp_init_config* c);
 int rcp_initialize_particles(const rcp_init_config* c, double* positions,
                              double* diameters, double* box_out, int8_t* walls_out,
                              double* diameter_scale_factor, double* phi_modifier);
+
+/* ------------------------------------------------------ surrogate network
+ * No counterpart in the reference: the learned warm start of batched runs.
+ * A fully connected network n_in -> n_hidden (x n_layers, ReLU) -> n_out
+ * (linear), FP32, trained and evaluated on the current device by this
+ * library's own kernels.  Parameters: one flat array, layer by layer, W
+ * (n_out_l x n_in_l, row-major) then b.  Loss: mean squared error over the
+ * batch and the outputs.  Sums run in a fixed order (reproducible). */
+typedef struct rcp_mlp rcp_mlp;
+int rcp_mlp_create(int n_in, int n_hidden, int n_layers, int n_out, uint64_t seed,
+                   rcp_mlp** out);
+void rcp_mlp_destroy(rcp_mlp* m);
+size_t rcp_mlp_num_params(const rcp_mlp* m);
+int rcp_mlp_get_params(const rcp_mlp* m, float* params);
+int rcp_mlp_set_params(rcp_mlp* m, const float* params);
+/* Y[B * n_out] = network(X[B * n_in]) */
+int rcp_mlp_predict(rcp_mlp* m, const float* X, size_t B, float* Y);
+/* loss and d loss / d params at (X, T[B * n_out]) */
+int rcp_mlp_gradients(rcp_mlp* m, const float* X, const float* T, size_t B, float* loss,
+                      float* grad);
+/* Adam (0.9, 0.999, 1e-8) over shuffled minibatches (seeded); losses[e]
+ * (may be NULL) = mean minibatch loss of epoch e */
+int rcp_mlp_train(rcp_mlp* m, const float* X, const float* T, size_t n, size_t batch,
+                  size_t epochs, float lr, uint64_t seed, float* losses);
 
 #ifdef __cplusplus
 }

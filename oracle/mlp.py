@@ -1,0 +1,81 @@
+"""numpy restatement of the surrogate network -- TEST INFRASTRUCTURE ONLY.
+
+The CPU oracle of the GPU network in ``rcpgenerator_amd/csrc/rcp_mlp.hip``
+(no counterpart in the reference): a fully connected ReLU network with a
+linear head, mean squared error loss and Adam, written from the textbook
+definitions in float64.  It is pinned by the tests in
+``tests/test_mlp_oracle.py``: gradients against central finite differences of
+``loss`` (which catch a dropped term, a transposed weight or a wrong ReLU
+mask) and Adam's first step against its closed form (the bias-corrected
+update of step one is lr * g / (|g| + eps)).  Only ``tests/`` import this
+module.
+
+Parameter layout (as the GPU network's): layer by layer, W (fan_out x
+fan_in, row-major) then b.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def shapes(n_in: int, n_hidden: int, n_layers: int, n_out: int):
+    """(fan_in, fan_out) per linear layer."""
+    dims = [n_in] + [n_hidden] * n_layers + [n_out]
+    return [(dims[k], dims[k + 1]) for k in range(len(dims) - 1)]
+
+
+def unpack(p, n_in, n_hidden, n_layers, n_out):
+    p = np.asarray(p, dtype=np.float64)
+    out, o = [], 0
+    for fi, fo in shapes(n_in, n_hidden, n_layers, n_out):
+        W = p[o:o + fi * fo].reshape(fo, fi)
+        o += fi * fo
+        b = p[o:o + fo]
+        o += fo
+        out.append((W, b))
+    assert o == p.size
+    return out
+
+
+def forward(p, X, n_in, n_hidden, n_layers, n_out):
+    """Outputs and the activations (input first) of the network at X."""
+    layers = unpack(p, n_in, n_hidden, n_layers, n_out)
+    A = [np.asarray(X, dtype=np.float64)]
+    for k, (W, b) in enumerate(layers):
+        Z = A[-1] @ W.T + b
+        A.append(np.maximum(Z, 0.0) if k < len(layers) - 1 else Z)
+    return A[-1], A
+
+
+def loss(p, X, T, *shape):
+    Y, _ = forward(p, X, *shape)
+    return float(np.mean((Y - np.asarray(T, dtype=np.float64)) ** 2))
+
+
+def gradients(p, X, T, n_in, n_hidden, n_layers, n_out):
+    """(loss, d loss / d p) by back-propagation."""
+    shape = (n_in, n_hidden, n_layers, n_out)
+    layers = unpack(p, *shape)
+    Y, A = forward(p, X, *shape)
+    E = Y - np.asarray(T, dtype=np.float64)
+    dZ = 2.0 * E / E.size
+    grads = []
+    for k in range(len(layers) - 1, -1, -1):
+        W, _ = layers[k]
+        grads.append((dZ.T @ A[k], dZ.sum(axis=0)))
+        if k > 0:
+            dZ = (dZ @ W) * (A[k] > 0.0)
+    flat = []
+    for dW, db in reversed(grads):
+        flat.append(dW.ravel())
+        flat.append(db)
+    return float(np.mean(E ** 2)), np.concatenate(flat)
+
+
+def adam_step(p, g, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam update (Kingma & Ba); returns new (p, m, v)."""
+    m = b1 * m + (1.0 - b1) * g
+    v = b2 * v + (1.0 - b2) * g * g
+    mh = m / (1.0 - b1 ** t)
+    vh = v / (1.0 - b2 ** t)
+    return p - lr * mh / (np.sqrt(vh) + eps), m, v

@@ -29,7 +29,9 @@ from ._rcpgenerator import (  # noqa: E402
     Engine,
     get_num_threads,
     initialize_particles,
+    last_batch_stats,
     run_packing,
+    run_packing_batch,
     run_packing_observed,
     set_num_threads,
 )
@@ -40,7 +42,9 @@ __all__ = [
     "Packing",
     "get_num_threads",
     "initialize_particles",
+    "last_batch_stats",
     "run_packing",
+    "run_packing_batch",
     "run_packing_observed",
     "set_num_threads",
 ]

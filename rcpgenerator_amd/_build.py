@@ -40,8 +40,8 @@ HOST_FLAGS = [
     "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__",
     "-I" + os.path.join(ROCM, "include"), "-I" + INC, "-I" + CSRC, "-Wall",
 ]
-HOST_SOURCES = ["rcp_engine.cpp", "rcp_capi.cpp", "rcp_init.cpp"]
-HIP_SOURCES = ["rcp_kernels.hip"]
+HOST_SOURCES = ["rcp_engine.cpp", "rcp_batch.cpp", "rcp_capi.cpp", "rcp_init.cpp"]
+HIP_SOURCES = ["rcp_kernels.hip", "rcp_mlp.hip"]
 
 
 def _run(cmd, verbose):

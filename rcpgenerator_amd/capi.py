@@ -29,9 +29,11 @@ SYMBOLS = (
     "rcp_engine_run", "rcp_engine_finish", "rcp_engine_sync", "rcp_engine_info",
     "rcp_engine_result", "rcp_engine_trace", "rcp_engine_stats", "rcp_engine_timing",
     "rcp_engine_destroy",
-    "rcp_run_packing_observed", "rcp_forces", "rcp_pairs", "rcp_mean_force",
+    "rcp_run_packing_observed", "rcp_run_batch", "rcp_batch_last_stats", "rcp_forces", "rcp_pairs", "rcp_mean_force",
     "rcp_adam_positions", "rcp_reorder_perm", "rcp_scale_diameters", "rcp_init_config_init",
-    "rcp_initialize_particles",
+    "rcp_initialize_particles", "rcp_mlp_create", "rcp_mlp_destroy", "rcp_mlp_num_params",
+    "rcp_mlp_get_params", "rcp_mlp_set_params", "rcp_mlp_predict", "rcp_mlp_gradients",
+    "rcp_mlp_train",
 )
 
 
@@ -51,6 +53,7 @@ class RunConfig(C.Structure):
         ("exact_reductions", C.c_int),
         ("time_kernels", C.c_int),
         ("neighbor_mode", C.c_int),
+        ("rung1_deadline", C.c_size_t),
     ]
 
 
@@ -60,6 +63,23 @@ class ResultInfo(C.Structure):
         ("phi", C.c_double), ("max_min_dist", C.c_double), ("force_magnitude", C.c_double),
         ("done", C.c_int), ("trace_len", C.c_size_t),
     ]
+
+
+class InitConfig(C.Structure):
+    _fields_ = [
+        ("phi", C.c_double), ("N", C.c_size_t), ("ndim", C.c_size_t),
+        ("box", C.c_void_p), ("nbox", C.c_size_t),
+        ("walls", C.c_void_p), ("nwalls", C.c_size_t),
+        ("fix_height", C.c_int), ("seed", C.c_uint64), ("dist_type", C.c_char_p),
+    ] + [(k, C.c_double) for k in ("d", "mu", "sigma", "mu1", "sigma1", "mu2", "sigma2", "p",
+                                     "d1", "d2", "d_min", "d_max", "exponent", "scale",
+                                     "shape")] + [
+        ("custom", C.c_void_p), ("ncustom", C.c_size_t),
+    ]
+
+
+BatchSink = C.CFUNCTYPE(None, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p)
+RCP_BATCH_POOL = 1
 
 
 class RcpError(RuntimeError):
@@ -103,6 +123,28 @@ def lib():
         L.rcp_reorder_perm.argtypes = [C.c_size_t, C.c_size_t, _dp, _dp, _u32p]
         L.rcp_scale_diameters.argtypes = [C.c_size_t, C.c_size_t, _dp, C.c_double, _dp,
                                           C.c_int, _dp, C.POINTER(C.c_double)]
+        L.rcp_init_config_init.argtypes = [C.POINTER(InitConfig)]
+        L.rcp_init_config_init.restype = None
+        L.rcp_initialize_particles.argtypes = [C.POINTER(InitConfig), _dp, _dp, _dp, _i8p,
+                                               C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+        L.rcp_mlp_create.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64,
+                                     C.POINTER(C.c_void_p)]
+        L.rcp_mlp_destroy.argtypes = [C.c_void_p]
+        L.rcp_mlp_destroy.restype = None
+        L.rcp_mlp_num_params.argtypes = [C.c_void_p]
+        L.rcp_mlp_num_params.restype = C.c_size_t
+        L.rcp_mlp_get_params.argtypes = [C.c_void_p, _fp]
+        L.rcp_mlp_set_params.argtypes = [C.c_void_p, _fp]
+        L.rcp_mlp_predict.argtypes = [C.c_void_p, _fp, C.c_size_t, _fp]
+        L.rcp_mlp_gradients.argtypes = [C.c_void_p, _fp, _fp, C.c_size_t,
+                                        C.POINTER(C.c_float), _fp]
+        L.rcp_mlp_train.argtypes = [C.c_void_p, _fp, _fp, C.c_size_t, C.c_size_t, C.c_size_t,
+                                    C.c_float, C.c_uint64, C.c_void_p]
+        L.rcp_run_batch.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, BatchSink,
+                                    C.c_void_p]
+        L.rcp_batch_last_stats.argtypes = [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                           C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -271,3 +313,146 @@ class Engine:
                "done": bool(i.done)}
         out.update(dict(zip(keys, H)))
         return out
+
+
+def run_batch(runs, threads: int = 0, pool: bool = False, diameters: bool = False):
+    """rcp_run_batch over runs = [(positions, diameters, box, walls, options)]
+    without building result dicts: per packing a dict of steps, phi,
+    max_min_dist, force_magnitude (and the final diameters when asked).
+    options keys: max_steps, exact_reductions, neighbor_mode, rung1_deadline, mu,
+    initial_phi_target.  Returns
+    (summaries, stats)."""
+    keep = []
+    cfgs = (RunConfig * len(runs))()
+    for k, (x, D, box, walls, opts) in enumerate(runs):
+        x, D = _f64(x), _f64(D)
+        box = _f64(box)
+        w = np.ascontiguousarray(walls, dtype=np.int8)
+        keep.append((x, D, box, w))
+        c = cfgs[k]
+        lib().rcp_run_config_init(C.byref(c))
+        c.N, c.ndim = x.shape
+        c.positions, c.diameters = x.ctypes.data, D.ctypes.data
+        c.box, c.nbox = box.ctypes.data, box.size
+        c.walls, c.nwalls = w.ctypes.data, w.size
+        opts = opts or {}
+        c.max_steps = int(opts.get("max_steps", 0))
+        c.exact_reductions = int(bool(opts.get("exact_reductions", False)))
+        c.neighbor_mode = int(opts.get("neighbor_mode", 0))
+        c.rung1_deadline = int(opts.get("rung1_deadline", 0))
+        if "mu" in opts:
+            c.has_mu, c.mu = 1, float(opts["mu"])
+        if "initial_phi_target" in opts:
+            c.has_initial_phi_target = 1
+            c.initial_phi_target = float(opts["initial_phi_target"])
+    out = [None] * len(runs)
+
+    def sink(_user, index, rc, e):
+        if rc != RCP_OK:
+            out[index] = {"rc": rc, "error": lib().rcp_last_error().decode()}
+            return
+        i = ResultInfo()
+        lib().rcp_engine_info(e, C.byref(i))
+        r = {"rc": rc, "N": i.N, "ndim": i.ndim, "steps": i.steps, "phi": i.phi,
+             "max_min_dist": i.max_min_dist, "force_magnitude": i.force_magnitude}
+        if diameters:
+            Dn = np.zeros(i.N)
+            lib().rcp_engine_result(e, None, Dn.ctypes.data, *([None] * 10))
+            r["diameters"] = Dn
+        out[index] = r
+
+    cb = BatchSink(sink)
+    rc = lib().rcp_run_batch(C.cast(cfgs, C.c_void_p), len(runs), int(threads),
+                             RCP_BATCH_POOL if pool else 0, cb, None)
+    rounds, steps, ms = C.c_uint64(), C.c_uint64(), C.c_double()
+    lib().rcp_batch_last_stats(C.byref(rounds), C.byref(steps), C.byref(ms))
+    if rc != RCP_OK and all(r is None or r.get("rc") == RCP_OK for r in out):
+        _check(rc)
+    return out, {"rounds": rounds.value, "packing_steps": steps.value, "device_ms": ms.value}
+
+
+def initialize(config: dict):
+    """rcp_initialize_particles with numpy outputs: (positions[N, ndim],
+    diameters[N], box[ndim], walls[ndim]).  config as initialize_particles
+    takes it (phi, N, Ndim, box, walls, seed, dist{...}); no custom lists."""
+    c = InitConfig()
+    lib().rcp_init_config_init(C.byref(c))
+    N, nd = int(config["N"]), int(config["Ndim"])
+    box = _f64(config.get("box", [1.0] * nd))
+    walls = np.ascontiguousarray(config.get("walls", [0] * nd), dtype=np.int8)
+    c.phi, c.N, c.ndim = float(config.get("phi", 0.05)), N, nd
+    c.box, c.nbox = box.ctypes.data, box.size
+    c.walls, c.nwalls = walls.ctypes.data, walls.size
+    c.fix_height = int(bool(config.get("fix_height", False)))
+    c.seed = int(config.get("seed", 0))
+    dist = dict(config.get("dist", {}))
+    name = str(dist.pop("type", "mono")).encode()
+    c.dist_type = name
+    for k, v in dist.items():
+        setattr(c, k, float(v))
+    x = np.zeros((N, nd))
+    D = np.zeros(N)
+    bo = np.zeros(nd)
+    wo = np.zeros(nd, dtype=np.int8)
+    f, pm = C.c_double(), C.c_double()
+    _check(lib().rcp_initialize_particles(C.byref(c), x, D, bo, wo, C.byref(f), C.byref(pm)))
+    return x, D, bo, wo
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Mlp:
+    """Owning handle over an rcp_mlp (the surrogate network on the GPU)."""
+
+    def __init__(self, n_in: int, n_hidden: int = 128, n_layers: int = 3, n_out: int = 2,
+                 seed: int = 0):
+        self.shape = (int(n_in), int(n_hidden), int(n_layers), int(n_out))
+        h = C.c_void_p()
+        _check(lib().rcp_mlp_create(*self.shape, int(seed), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib().rcp_mlp_destroy(h)
+            self._h = None
+
+    @property
+    def num_params(self) -> int:
+        return int(lib().rcp_mlp_num_params(self._h))
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.num_params, dtype=np.float32)
+        _check(lib().rcp_mlp_get_params(self._h, p))
+        return p
+
+    def set_params(self, p) -> None:
+        p = _f32(p)
+        if p.size != self.num_params:
+            raise ValueError(f"expected {self.num_params} parameters, got {p.size}")
+        _check(lib().rcp_mlp_set_params(self._h, p))
+
+    def predict(self, X) -> np.ndarray:
+        X = _f32(X).reshape(-1, self.shape[0])
+        Y = np.zeros((X.shape[0], self.shape[3]), dtype=np.float32)
+        _check(lib().rcp_mlp_predict(self._h, X, X.shape[0], Y))
+        return Y
+
+    def gradients(self, X, T):
+        X = _f32(X).reshape(-1, self.shape[0])
+        T = _f32(T).reshape(-1, self.shape[3])
+        g = np.zeros(self.num_params, dtype=np.float32)
+        loss = C.c_float()
+        _check(lib().rcp_mlp_gradients(self._h, X, T, X.shape[0], C.byref(loss), g))
+        return float(loss.value), g
+
+    def train(self, X, T, batch: int = 256, epochs: int = 100, lr: float = 1e-3,
+              seed: int = 0) -> np.ndarray:
+        X = _f32(X).reshape(-1, self.shape[0])
+        T = _f32(T).reshape(-1, self.shape[3])
+        losses = np.zeros(epochs, dtype=np.float32)
+        _check(lib().rcp_mlp_train(self._h, X, T, X.shape[0], int(batch), int(epochs),
+                                   float(lr), int(seed), losses.ctypes.data))
+        return losses

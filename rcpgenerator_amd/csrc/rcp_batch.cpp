@@ -1,0 +1,587 @@
+// Batched engine: independent packings of one dimension step together, the
+// device state of all of them in one set of arrays and every step one
+// launch per kernel for the whole set (the reference runs such batches as a
+// process pool of single-threaded engines, rcptools/jobs.py:392-401,
+// 531-557).  Each packing keeps its own host controller (Engine in hosted
+// mode: schedule, histories, size-class neighbour decisions) and its own
+// slices of the arrays; the kernels give every packing the partial rows and
+// fold of its single run in size-class mode, so a batched packing follows
+// its single run bit for bit.  Packings leave the launches as they finish.
+#include "rcp_batch.h"
+
+#include "rcp_kernels.h"
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+namespace rcp {
+
+namespace {
+
+constexpr size_t kAlign = 256;
+
+size_t align_up(size_t n) { return (n + kAlign - 1) / kAlign * kAlign; }
+
+// Pinned host buffer (one allocation, grown on demand).
+struct Pinned {
+    char* p = nullptr;
+    size_t bytes = 0;
+    ~Pinned()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+    void ensure(size_t n)
+    {
+        if (n <= bytes) return;
+        if (p) hip_check(hipHostFree(p), "hipHostFree");
+        p = nullptr;
+        hip_check(hipHostMalloc((void**)&p, n, hipHostMallocDefault), "hipHostMalloc");
+        bytes = n;
+    }
+};
+
+struct DevBuf {
+    char* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { dev_free(p); }
+    void ensure(size_t n)
+    {
+        if (n <= bytes) return;
+        dev_free(p);
+        p = nullptr;
+        p = static_cast<char*>(dev_alloc(n));
+        bytes = n;
+    }
+};
+
+// One packing of a group: its controller and its slices of the arrays.
+struct Member {
+    std::size_t index = 0;             // caller's index
+    std::unique_ptr<Engine> ctl;
+    int N = 0;
+    size_t poff = 0, coff = 0;         // first particle / cell-table entry
+    long long cells = 0;               // cell-table capacity (entries - 1)
+    size_t row0 = 0;                   // first partial row
+    int nb = 0;                        // workgroups of the per-thread step
+    int wide_blocks = 0;               // workgroups of the one-wave step
+    int cx = 0, cm = 0, cd = 0, co = 0;   // current buffer of x, (m, v), (D0, tag), origin
+    MultiGrid mg;
+    Engine::StepPlan plan;
+    bool rebin = false;
+};
+
+class Group {
+public:
+    Group(int ndim, std::vector<Member>&& members, hipStream_t s)
+        : nd_(ndim), m_(std::move(members)), s_(s)
+    {
+        allocate();
+        upload();
+    }
+    Group(const Group&) = delete;
+    Group& operator=(const Group&) = delete;
+
+    bool active() const { return !live_.empty(); }
+    // first force evaluation of every member (RG:5011-5013)
+    void initial(const BatchSink& done);
+    // one step of every running member: plan, reorders, binning, launches
+    // (finished members are handed to `done` first); false when none runs
+    bool launch(const BatchSink& done);
+    // wait for the launched step and complete it on the controllers
+    void complete();
+    double gpu_ms() const { return gpu_ms_; }
+    uint64_t member_steps() const { return member_steps_; }
+
+private:
+    // device arrays (each the concatenation of the members' slices)
+    double* x_[2] = {};
+    double* m_arr_[2] = {};
+    double* v_[2] = {};
+    double* anchor_ = nullptr;
+    double* D0_[2] = {};
+    uint32_t* tag_[2] = {};
+    uint32_t *inv_tag_ = nullptr, *tag_tmp_ = nullptr, *cell_of_ = nullptr, *slot_ = nullptr,
+             *perm_ = nullptr, *cell_count_ = nullptr, *cell_start_ = nullptr;
+    unsigned long long* origin_[2] = {};
+    double* partials_ = nullptr;
+    double* results_ = nullptr;
+    double* tmp_ = nullptr;            // largest member's N * ndim (result scatter)
+    double* tmp2_ = nullptr;           // largest member's N
+    SortTmp sort_;                     // reference reorders, one member at a time
+    DevBuf arena_;
+    // per-step tables: host staging and device copy with the same layout
+    Pinned h_tab_;
+    DevBuf d_tab_;
+    Pinned h_res_;
+    size_t tab_bytes_ = 0;
+    size_t off_packs_ = 0, off_bins_ = 0, off_u32_ = 0;
+
+    int nd_;
+    std::vector<Member> m_;
+    std::vector<int> live_;            // running members, launch order
+    std::vector<int> stepping_;        // members of the launched step
+    hipStream_t s_;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    double gpu_ms_ = 0.0;
+    uint64_t member_steps_ = 0;
+
+    void allocate();
+    void upload();
+    void reorder(Member& M);
+    void finish(Member& M, const BatchSink& done);
+    void stage_and_launch(bool do_adam);
+};
+
+void Group::allocate()
+{
+    size_t NT = 0, CT = 0, rows = 0, maxN = 0;
+    for (Member& M : m_) {
+        M.poff = NT;
+        M.coff = CT;
+        M.cells = M.ctl->cell_budget();
+        NT += (size_t)M.N;
+        CT += (size_t)M.cells + 1;
+        M.nb = (int)((M.N + kBatchStepBlock - 1) / kBatchStepBlock);
+        const long long wb = M.ctl->wide_bound();
+        M.wide_blocks = wb > 0 ? (int)std::min<long long>(kBatchWideGrid,
+                                                          (wb + kBatchWideWaves - 1) / kBatchWideWaves)
+                               : 0;
+        M.row0 = rows;
+        rows += (size_t)M.nb + (size_t)M.wide_blocks;
+        maxN = std::max(maxN, (size_t)M.N);
+    }
+    const size_t d = (size_t)nd_;
+    size_t bytes = 0;
+    auto take = [&](size_t n) {
+        const size_t o = bytes;
+        bytes += align_up(n);
+        return o;
+    };
+    size_t ox[2], om[2], ov[2], oD[2], ot[2], oo[2];
+    for (int k = 0; k < 2; ++k) {
+        ox[k] = take(NT * d * sizeof(double));
+        om[k] = take(NT * d * sizeof(double));
+        ov[k] = take(NT * d * sizeof(double));
+        oD[k] = take(NT * sizeof(double));
+        ot[k] = take(NT * sizeof(uint32_t));
+        oo[k] = take(NT * sizeof(unsigned long long));
+    }
+    const size_t oa = take(NT * d * sizeof(double));
+    const size_t oinv = take(NT * sizeof(uint32_t)), otmp = take(NT * sizeof(uint32_t)),
+                 ocof = take(NT * sizeof(uint32_t)), oslot = take(NT * sizeof(uint32_t)),
+                 operm = take(NT * sizeof(uint32_t));
+    const size_t occ = take(CT * sizeof(uint32_t)), ocs = take(CT * sizeof(uint32_t));
+    const size_t opart = take(rows * S_NUM * sizeof(double));
+    const size_t ores = take(m_.size() * S_NUM * sizeof(double));
+    const size_t otx = take(maxN * d * sizeof(double)), otx2 = take(maxN * sizeof(double));
+    const size_t ok = take(maxN * sizeof(unsigned long long)),
+                 oka = take(maxN * sizeof(unsigned long long)),
+                 ovl = take(maxN * sizeof(uint32_t)), ovla = take(maxN * sizeof(uint32_t)),
+                 oh = take(sort_hist_entries(maxN) * sizeof(uint32_t)),
+                 osc = take(scan_tmp_bytes(std::max(sort_hist_entries(maxN), maxN) + 8));
+    arena_.ensure(bytes);
+    char* b = arena_.p;
+    for (int k = 0; k < 2; ++k) {
+        x_[k] = reinterpret_cast<double*>(b + ox[k]);
+        m_arr_[k] = reinterpret_cast<double*>(b + om[k]);
+        v_[k] = reinterpret_cast<double*>(b + ov[k]);
+        D0_[k] = reinterpret_cast<double*>(b + oD[k]);
+        tag_[k] = reinterpret_cast<uint32_t*>(b + ot[k]);
+        origin_[k] = reinterpret_cast<unsigned long long*>(b + oo[k]);
+    }
+    anchor_ = reinterpret_cast<double*>(b + oa);
+    inv_tag_ = reinterpret_cast<uint32_t*>(b + oinv);
+    tag_tmp_ = reinterpret_cast<uint32_t*>(b + otmp);
+    cell_of_ = reinterpret_cast<uint32_t*>(b + ocof);
+    slot_ = reinterpret_cast<uint32_t*>(b + oslot);
+    perm_ = reinterpret_cast<uint32_t*>(b + operm);
+    cell_count_ = reinterpret_cast<uint32_t*>(b + occ);
+    cell_start_ = reinterpret_cast<uint32_t*>(b + ocs);
+    partials_ = reinterpret_cast<double*>(b + opart);
+    results_ = reinterpret_cast<double*>(b + ores);
+    tmp_ = reinterpret_cast<double*>(b + otx);
+    tmp2_ = reinterpret_cast<double*>(b + otx2);
+    sort_.keys = reinterpret_cast<unsigned long long*>(b + ok);
+    sort_.keys_alt = reinterpret_cast<unsigned long long*>(b + oka);
+    sort_.vals = reinterpret_cast<uint32_t*>(b + ovl);
+    sort_.vals_alt = reinterpret_cast<uint32_t*>(b + ovla);
+    sort_.hist = reinterpret_cast<uint32_t*>(b + oh);
+    sort_.scan_tmp = b + osc;
+
+    // table layout: packs, bins, then five u32 arrays of P + 1
+    const size_t P = m_.size();
+    off_packs_ = 0;
+    off_bins_ = align_up(P * sizeof(BatchPack));
+    off_u32_ = off_bins_ + align_up(P * sizeof(BatchBin));
+    tab_bytes_ = off_u32_ + 5 * align_up((P + 1) * sizeof(uint32_t));
+    h_tab_.ensure(tab_bytes_);
+    d_tab_.ensure(tab_bytes_);
+    h_res_.ensure(P * S_NUM * sizeof(double));
+    hip_check(hipEventCreate(&ev0_), "hipEventCreate");
+    hip_check(hipEventCreate(&ev1_), "hipEventCreate");
+}
+
+void Group::upload()
+{
+    size_t NT = 0;
+    for (const Member& M : m_) NT += (size_t)M.N;
+    const size_t d = (size_t)nd_;
+    std::vector<double> x(NT * d), D(NT);
+    std::vector<uint32_t> tag(NT);
+    std::vector<unsigned long long> org(NT);
+    for (const Member& M : m_) {
+        const auto& xs = M.ctl->start_positions();
+        const auto& Ds = M.ctl->start_diameters();
+        std::copy(xs.begin(), xs.end(), x.begin() + M.poff * d);
+        std::copy(Ds.begin(), Ds.end(), D.begin() + M.poff);
+        for (int i = 0; i < M.N; ++i) {
+            tag[M.poff + i] = (uint32_t)i;
+            org[M.poff + i] = (unsigned long long)i;
+        }
+    }
+    hip_check(hipMemcpyAsync(x_[0], x.data(), x.size() * sizeof(double), hipMemcpyHostToDevice,
+                             s_), "upload x");
+    hip_check(hipMemcpyAsync(D0_[0], D.data(), D.size() * sizeof(double), hipMemcpyHostToDevice,
+                             s_), "upload D0");
+    hip_check(hipMemcpyAsync(tag_[0], tag.data(), tag.size() * sizeof(uint32_t),
+                             hipMemcpyHostToDevice, s_), "upload tags");
+    hip_check(hipMemcpyAsync(origin_[0], org.data(), org.size() * sizeof(unsigned long long),
+                             hipMemcpyHostToDevice, s_), "upload origin");
+    hip_check(hipMemsetAsync(m_arr_[0], 0, NT * d * sizeof(double), s_), "zero m");
+    hip_check(hipMemsetAsync(v_[0], 0, NT * d * sizeof(double), s_), "zero v");
+    hip_check(hipStreamSynchronize(s_), "upload sync");
+    live_.resize(m_.size());
+    std::iota(live_.begin(), live_.end(), 0);
+}
+
+// The reference's spatial reorder of one member (RG:4850-4948): tags only,
+// origin carried along in reference order.
+void Group::reorder(Member& M)
+{
+    const size_t p = M.poff;
+    Geometry g = M.ctl->geometry();
+    uint32_t* tag = tag_[M.cd] + p;
+    if (launch_inv_tag(tag, inv_tag_ + p, M.N, s_)) throw HipFailure("inv_tag launch");
+    if (launch_reference_reorder(g, x_[M.cx] + p * nd_, tag, inv_tag_ + p, tag_tmp_ + p, sort_,
+                                 s_))
+        throw HipFailure("reference reorder launch");
+    if (launch_rank_scatter_u64(origin_[M.co] + p, origin_[1 - M.co] + p, tag_tmp_ + p, M.N, s_))
+        throw HipFailure("origin scatter launch");
+    M.co ^= 1;
+    M.ctl->count_reorder();
+}
+
+void Group::finish(Member& M, const BatchSink& done)
+{
+    const size_t p = M.poff;
+    const int N = M.N;
+    Engine::HostedLaunch l;
+    M.ctl->hosted_launch(l);
+    std::vector<double> x((size_t)N * nd_), D(N);
+    std::vector<uint64_t> o(N);
+    const uint32_t* tag = tag_[M.cd] + p;
+    if (launch_to_ref_f64(x_[M.cx] + p * nd_, tmp_, tag, N, nd_, s_))
+        throw HipFailure("scatter launch");
+    hip_check(hipMemcpyAsync(x.data(), tmp_, x.size() * sizeof(double), hipMemcpyDeviceToHost, s_),
+              "download");
+    if (launch_diameters(D0_[M.cd] + p, nullptr, l.kappa, tmp2_, N, s_))
+        throw HipFailure("diameters launch");
+    hip_check(hipStreamSynchronize(s_), "sync");
+    if (launch_to_ref_f64(tmp2_, tmp_, tag, N, 1, s_)) throw HipFailure("scatter launch");
+    hip_check(hipMemcpyAsync(D.data(), tmp_, D.size() * sizeof(double), hipMemcpyDeviceToHost, s_),
+              "download");
+    hip_check(hipMemcpyAsync(o.data(), origin_[M.co] + p, o.size() * sizeof(uint64_t),
+                             hipMemcpyDeviceToHost, s_), "download");
+    hip_check(hipStreamSynchronize(s_), "sync");
+    M.ctl->set_final_state(std::move(x), std::move(D), std::move(o));
+    done(M.index, RCP_OK, std::move(M.ctl));
+}
+
+// Tables of the members in stepping_ (packs; bins of those that re-bin)
+// and the launches.
+void Group::stage_and_launch(bool do_adam)
+{
+    const int n = (int)stepping_.size();
+    char* h = h_tab_.p;
+    BatchPack* packs = reinterpret_cast<BatchPack*>(h + off_packs_);
+    BatchBin* bins = reinterpret_cast<BatchBin*>(h + off_bins_);
+    const size_t u32n = align_up((m_.size() + 1) * sizeof(uint32_t));
+    unsigned* step_pre = reinterpret_cast<unsigned*>(h + off_u32_);
+    unsigned* wide_pre = reinterpret_cast<unsigned*>(h + off_u32_ + u32n);
+    int* wide_ids = reinterpret_cast<int*>(h + off_u32_ + 2 * u32n);
+    unsigned* part_pre = reinterpret_cast<unsigned*>(h + off_u32_ + 3 * u32n);
+    unsigned* cell_pre = reinterpret_cast<unsigned*>(h + off_u32_ + 4 * u32n);
+    int nbins = 0, nwide = 0;
+    unsigned sb = 0, wb = 0, pb = 0, cb = 0;
+    for (int e = 0; e < n; ++e) {
+        Member& M = m_[stepping_[e]];
+        const size_t p = M.poff, pd = M.poff * nd_;
+        const Geometry& g = M.ctl->geometry();
+        Engine::HostedLaunch l;
+        M.ctl->hosted_launch(l);
+        if (M.rebin) {
+            BatchBin& B = bins[nbins];
+            B = BatchBin();
+            B.b.g = g;
+            B.b.grid = M.mg.grid[0];
+            B.b.grid.total = M.mg.base[M.mg.C];
+            B.b.kappa = l.kappa;
+            B.b.x = x_[M.cx] + pd;
+            B.b.m = m_arr_[M.cm] + pd;
+            B.b.v = v_[M.cm] + pd;
+            B.b.D0 = D0_[M.cd] + p;
+            B.b.tag = tag_[M.cd] + p;
+            B.b.x_o = x_[1 - M.cx] + pd;
+            B.b.m_o = m_arr_[1 - M.cm] + pd;
+            B.b.v_o = v_[1 - M.cm] + pd;
+            B.b.D0_o = D0_[1 - M.cd] + p;
+            B.b.tag_o = tag_[1 - M.cd] + p;
+            B.b.cell_of = cell_of_ + p;
+            B.b.slot = slot_ + p;
+            B.b.cell_count = cell_count_ + M.coff;
+            B.b.cell_start = cell_start_ + M.coff;
+            B.b.perm = perm_ + p;
+            B.mg = M.mg;
+            B.anchor_o = l.anchor ? anchor_ + pd : nullptr;
+            part_pre[nbins] = pb;
+            cell_pre[nbins] = cb;
+            pb += (unsigned)((M.N + kBatchBinBlock - 1) / kBatchBinBlock);
+            cb += (unsigned)((B.b.grid.total + 1 + kBatchBinBlock - 1) / kBatchBinBlock);
+            ++nbins;
+            M.cx ^= 1;
+            M.cm ^= 1;
+            M.cd ^= 1;
+            M.rebin = false;
+        }
+        BatchPack& P = packs[e];
+        P = BatchPack();
+        StepArgs& a = P.a;
+        a.g = g;
+        a.x_in = x_[M.cx] + pd;
+        a.x_out = x_[1 - M.cx] + pd;
+        a.m = m_arr_[M.cm] + pd;
+        a.v = v_[M.cm] + pd;
+        a.D0 = D0_[M.cd] + p;
+        a.tag = tag_[M.cd] + p;
+        a.x_anchor = l.anchor ? anchor_ + pd : nullptr;
+        a.cell_start = cell_start_ + M.coff;
+        a.kappa = l.kappa;
+        a.alpha = do_adam ? l.alpha : 0.0;
+        a.inv_b1 = do_adam ? M.plan.inv_b1 : 1.0;
+        a.inv_b2 = do_adam ? M.plan.inv_b2 : 1.0;
+        a.do_adam = do_adam ? 1 : 0;
+        a.partials = partials_ + M.row0 * S_NUM;
+        a.tile_len = M.nb;
+        a.pad = l.pad;
+        P.mg = M.mg;
+        P.mg.grow = l.grow;
+        P.wide_blocks = M.mg.wide > 0 ? std::max(1, M.wide_blocks) : 0;
+        P.rows = M.nb + P.wide_blocks;
+        step_pre[e] = sb;
+        sb += (unsigned)M.nb;
+        if (P.wide_blocks > 0) {
+            wide_ids[nwide] = e;
+            wide_pre[nwide] = wb;
+            wb += (unsigned)P.wide_blocks;
+            ++nwide;
+        }
+    }
+    step_pre[n] = sb;
+    wide_pre[nwide] = wb;
+    part_pre[nbins] = pb;
+    cell_pre[nbins] = cb;
+    // one upload of the used parts of the table block
+    char* d = d_tab_.p;
+    hip_check(hipMemcpyAsync(d + off_packs_, h + off_packs_, n * sizeof(BatchPack),
+                             hipMemcpyHostToDevice, s_), "upload packs");
+    if (nbins)
+        hip_check(hipMemcpyAsync(d + off_bins_, h + off_bins_, nbins * sizeof(BatchBin),
+                                 hipMemcpyHostToDevice, s_), "upload bins");
+    hip_check(hipMemcpyAsync(d + off_u32_, h + off_u32_, 5 * u32n, hipMemcpyHostToDevice, s_),
+              "upload tables");
+    const BatchPack* dpacks = reinterpret_cast<const BatchPack*>(d + off_packs_);
+    const BatchBin* dbins = reinterpret_cast<const BatchBin*>(d + off_bins_);
+    auto table = [&](size_t k, int cnt, unsigned blocks, bool ids) {
+        BatchTable t;
+        t.prefix = reinterpret_cast<const unsigned*>(d + off_u32_ + k * u32n);
+        t.ids = ids ? reinterpret_cast<const int*>(d + off_u32_ + 2 * u32n) : nullptr;
+        t.n = cnt;
+        t.blocks = blocks;
+        return t;
+    };
+    hip_check(hipEventRecord(ev0_, s_), "event");
+    if (nbins) {
+        if (launch_batch_rebin(nd_, dbins, nbins, table(3, nbins, pb, false),
+                               table(4, nbins, cb, false), s_))
+            throw HipFailure("batched rebin launch");
+    }
+    if (launch_batch_step(nd_, dpacks, n, table(0, n, sb, false), table(1, nwide, wb, true),
+                          results_, s_))
+        throw HipFailure("batched step launch");
+    hip_check(hipEventRecord(ev1_, s_), "event");
+    hip_check(hipMemcpyAsync(h_res_.p, results_, (size_t)n * S_NUM * sizeof(double),
+                             hipMemcpyDeviceToHost, s_), "results download");
+}
+
+void Group::initial(const BatchSink& done)
+{
+    (void)done;
+    stepping_ = live_;
+    for (int k : stepping_) {
+        Member& M = m_[k];
+        // initial reorder (RG:4990-4993) before the first binning
+        if (M.ctl->reorder_interval() > 0) reorder(M);
+        M.ctl->hosted_initial(M.mg);
+        M.rebin = true;
+    }
+    stage_and_launch(false);
+    hip_check(hipStreamSynchronize(s_), "initial sync");
+    const double* r = reinterpret_cast<const double*>(h_res_.p);
+    for (size_t e = 0; e < stepping_.size(); ++e) {
+        Engine::Scalars sc;
+        std::memcpy(sc.v, r + e * S_NUM, sizeof(sc.v));
+        m_[stepping_[e]].ctl->complete_initial(sc);
+    }
+    stepping_.clear();
+}
+
+bool Group::launch(const BatchSink& done)
+{
+    stepping_.clear();
+    std::vector<int> keep;
+    keep.reserve(live_.size());
+    for (int k : live_) {
+        Member& M = m_[k];
+        if (M.ctl->done() || !M.ctl->plan_step(M.plan)) {
+            finish(M, done);
+            continue;
+        }
+        keep.push_back(k);
+    }
+    live_.swap(keep);
+    if (live_.empty()) return false;
+    for (int k : live_) {
+        Member& M = m_[k];
+        // periodic reorder (RG:5661-5663) before the neighbour decision
+        if (M.plan.reorder) reorder(M);
+        M.rebin = M.ctl->hosted_binning(M.mg);
+    }
+    stepping_ = live_;
+    stage_and_launch(true);
+    return true;
+}
+
+void Group::complete()
+{
+    if (stepping_.empty()) return;
+    hip_check(hipStreamSynchronize(s_), "step sync");
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ev0_, ev1_) == hipSuccess) gpu_ms_ += ms;
+    const double* r = reinterpret_cast<const double*>(h_res_.p);
+    for (size_t e = 0; e < stepping_.size(); ++e) {
+        Member& M = m_[stepping_[e]];
+        Engine::Scalars sc;
+        std::memcpy(sc.v, r + e * S_NUM, sizeof(sc.v));
+        M.cx ^= 1;   // the step wrote the new positions to the other buffer
+        M.ctl->hosted_complete(sc, M.plan);
+    }
+    member_steps_ += stepping_.size();
+    stepping_.clear();
+}
+
+}  // namespace
+
+bool batch_eligible(const rcp_run_config& c)
+{
+    if (c.exact_reductions || c.fix_height || c.has_target_phi) return false;
+    if (c.neighbor_mode != 0 && c.neighbor_mode != 4) return false;
+    if (c.ndim < 1 || c.ndim > (size_t)kMaxDim || c.N == 0) return false;
+    if (c.fixed)
+        for (std::size_t i = 0; i < c.N; ++i)
+            if (c.fixed[i]) return false;
+    return true;
+}
+
+BatchStats run_batched(const rcp_run_config* cfgs, const std::vector<size_t>& idx,
+                       const BatchSink& sink)
+{
+    BatchStats st;
+    // every packing is reported exactly once, also when a group fails
+    std::vector<size_t> pending(idx);
+    std::sort(pending.begin(), pending.end());
+    std::vector<char> reported(pending.size(), 0);
+    const BatchSink done = [&](std::size_t i, int rc, std::unique_ptr<Engine> e) {
+        const auto it = std::lower_bound(pending.begin(), pending.end(), i);
+        reported[it - pending.begin()] = 1;
+        sink(i, rc, std::move(e));
+    };
+    // controllers (set-up errors are reported per packing)
+    std::vector<std::vector<Member>> by_dim(kMaxDim + 1);
+    for (size_t i : idx) {
+        Member M;
+        M.index = i;
+        try {
+            M.ctl.reset(new Engine(cfgs[i], Engine::Hosted{}));
+        } catch (const ArgError& ex) {
+            g_last_error = ex.what();
+            done(i, RCP_E_ARG, nullptr);
+            continue;
+        } catch (const RtError& ex) {
+            g_last_error = ex.what();
+            done(i, RCP_E_RUNTIME, nullptr);
+            continue;
+        }
+        M.N = (int)cfgs[i].N;
+        by_dim[cfgs[i].ndim].push_back(std::move(M));
+    }
+    std::vector<hipStream_t> streams;
+    std::vector<std::unique_ptr<Group>> groups;
+    int rc = RCP_OK;
+    try {
+        for (int d = 1; d <= kMaxDim; ++d) {
+            if (by_dim[d].empty()) continue;
+            hipStream_t s = nullptr;
+            hip_check(hipStreamCreate(&s), "hipStreamCreate");
+            streams.push_back(s);
+            groups.emplace_back(new Group(d, std::move(by_dim[d]), s));
+        }
+        for (auto& g : groups) g->initial(done);
+        bool any = true;
+        while (any) {
+            any = false;
+            for (auto& g : groups)
+                if (g->active() && g->launch(done)) any = true;
+            for (auto& g : groups) g->complete();
+            if (any) ++st.steps;
+        }
+    } catch (const HipFailure& ex) {
+        g_last_error = ex.what();
+        rc = RCP_E_HIP;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "out of memory";
+        rc = RCP_E_NOMEM;
+    } catch (const std::exception& ex) {
+        g_last_error = ex.what();
+        rc = RCP_E_RUNTIME;
+    }
+    if (rc != RCP_OK) {
+        const std::string msg = g_last_error;
+        for (size_t k = 0; k < pending.size(); ++k) {
+            if (reported[k]) continue;
+            g_last_error = msg;
+            done(pending[k], rc, nullptr);
+        }
+        g_last_error = msg;
+        st.rc = rc;
+    }
+    for (auto& g : groups) {
+        st.gpu_ms += g->gpu_ms();
+        st.member_steps += g->member_steps();
+    }
+    groups.clear();
+    for (hipStream_t s : streams) (void)hipStreamDestroy(s);
+    return st;
+}
+
+}  // namespace rcp

@@ -1,0 +1,33 @@
+// Batched engine: many independent packings advanced by shared launches.
+#pragma once
+
+#include "rcp_engine.h"
+
+#include <functional>
+#include <memory>
+#include <vector>
+
+namespace rcp {
+
+// Called once per packing when it has finished (rc RCP_OK: the hosted
+// controller with the final state) or failed (rc an RCP_E_* code, engine
+// null, g_last_error set).
+using BatchSink = std::function<void(std::size_t index, int rc, std::unique_ptr<Engine> e)>;
+
+struct BatchStats {
+    int rc = RCP_OK;
+    uint64_t steps = 0;          // batched steps (launch rounds)
+    uint64_t member_steps = 0;   // packing-steps
+    double gpu_ms = 0.0;         // device time of the re-binning + step launches
+};
+
+// Runs the batched kernels support: fast reductions, neighbour mode 0 or 4,
+// no frozen diameters, fix_height or target_phi.
+bool batch_eligible(const rcp_run_config& c);
+
+// Runs the packings cfgs[i], i in idx, to completion on the current device,
+// grouped by dimension; `done` is called on this thread for each.
+BatchStats run_batched(const rcp_run_config* cfgs, const std::vector<size_t>& idx,
+                       const BatchSink& done);
+
+}  // namespace rcp

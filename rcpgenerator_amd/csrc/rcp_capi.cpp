@@ -2,16 +2,20 @@
 // Exceptions never cross this boundary; they become RCP_E* codes with the
 // message kept in a thread-local string.
 #include "rcp_amd.h"
+#include "rcp_batch.h"
 #include "rcp_engine.h"
 #include "rcp_kernels.h"
+#include "rcp_mlp.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 struct rcp_engine {
@@ -19,6 +23,8 @@ struct rcp_engine {
 };
 
 namespace {
+
+thread_local rcp::BatchStats g_batch_stats;
 
 template <class F>
 int guarded(F&& f)
@@ -38,6 +44,9 @@ int guarded(F&& f)
     } catch (const std::bad_alloc&) {
         rcp::g_last_error = "out of memory";
         return RCP_E_NOMEM;
+    } catch (const std::invalid_argument& ex) {
+        rcp::g_last_error = ex.what();
+        return RCP_E_ARG;
     } catch (const std::exception& ex) {
         rcp::g_last_error = ex.what();
         return RCP_E_RUNTIME;
@@ -301,6 +310,73 @@ int rcp_run_packing_observed(const rcp_run_config* cfg, size_t progress_interval
     return RCP_OK;
 }
 
+int rcp_run_batch(const rcp_run_config* cfgs, size_t count, int threads, int flags,
+                  rcp_batch_sink sink, void* user)
+{
+    return guarded([&]() -> int {
+        g_batch_stats = rcp::BatchStats();
+        if (count == 0) return RCP_OK;
+        if (!cfgs) throw rcp::ArgError("rcp: null config array");
+        int dev = 0;
+        rcp::hip_check(hipGetDevice(&dev), "hipGetDevice");
+        std::vector<size_t> batched, pooled;
+        for (size_t i = 0; i < count; ++i) {
+            if (!(flags & RCP_BATCH_POOL) && rcp::batch_eligible(cfgs[i])) batched.push_back(i);
+            else pooled.push_back(i);
+        }
+        std::atomic<int> first_rc{RCP_OK};
+        auto note = [&](int rc) {
+            int expect = RCP_OK;
+            if (rc != RCP_OK) first_rc.compare_exchange_strong(expect, rc);
+        };
+        // engine pool: largest first, so the long runs start early
+        std::stable_sort(pooled.begin(), pooled.end(), [&](size_t a, size_t b) {
+            return cfgs[a].N * cfgs[a].ndim > cfgs[b].N * cfgs[b].ndim;
+        });
+        const size_t nthreads =
+            std::min<size_t>(pooled.size(), threads > 0 ? (size_t)threads : 8);
+        std::atomic<size_t> next{0};
+        auto worker = [&]() {
+            (void)hipSetDevice(dev);
+            for (size_t k; (k = next.fetch_add(1)) < pooled.size();) {
+                const size_t i = pooled[k];
+                rcp_engine* e = nullptr;
+                const int rc = rcp_run_packing_observed(&cfgs[i], 0, 0, 0, nullptr, nullptr, &e);
+                note(rc);
+                if (sink) sink(user, i, rc, e);
+                rcp_engine_destroy(e);
+            }
+        };
+        std::vector<std::thread> pool;
+        pool.reserve(nthreads);
+        for (size_t t = 0; t < nthreads; ++t) pool.emplace_back(worker);
+        if (!batched.empty()) {
+            const rcp::BatchSink done = [&](size_t i, int rc, std::unique_ptr<rcp::Engine> eng) {
+                note(rc);
+                if (!sink) return;
+                if (rc == RCP_OK) {
+                    rcp_engine h;
+                    h.e = std::move(eng);
+                    sink(user, i, rc, &h);
+                } else {
+                    sink(user, i, rc, nullptr);
+                }
+            };
+            g_batch_stats = rcp::run_batched(cfgs, batched, done);
+        }
+        for (auto& th : pool) th.join();
+        return first_rc.load();
+    });
+}
+
+int rcp_batch_last_stats(uint64_t* rounds, uint64_t* packing_steps, double* device_ms)
+{
+    if (rounds) *rounds = g_batch_stats.steps;
+    if (packing_steps) *packing_steps = g_batch_stats.member_steps;
+    if (device_ms) *device_ms = g_batch_stats.gpu_ms;
+    return RCP_OK;
+}
+
 // --------------------------------------------------------- kernel level
 
 int rcp_forces(size_t N, size_t ndim, const double* x, const double* D, const double* box,
@@ -492,6 +568,75 @@ int rcp_scale_diameters(size_t N, size_t ndim, const double* D, double phi_targe
         const double f = rcp::scale_diameters(Dv, phi_target, bv, ndim, fix_height != 0, o);
         std::memcpy(out, o.data(), N * sizeof(double));
         if (factor) *factor = f;
+        return RCP_OK;
+    });
+}
+
+// ------------------------------------------------------ surrogate network
+
+struct rcp_mlp {
+    std::unique_ptr<rcp::Mlp> m;
+};
+
+int rcp_mlp_create(int n_in, int n_hidden, int n_layers, int n_out, uint64_t seed,
+                   rcp_mlp** out)
+{
+    *out = nullptr;
+    return guarded([&]() {
+        std::unique_ptr<rcp_mlp> h(new rcp_mlp);
+        h->m.reset(new rcp::Mlp(n_in, n_hidden, n_layers, n_out, seed));
+        *out = h.release();
+        return RCP_OK;
+    });
+}
+
+void rcp_mlp_destroy(rcp_mlp* m) { delete m; }
+
+size_t rcp_mlp_num_params(const rcp_mlp* m) { return m ? m->m->num_params() : 0; }
+
+int rcp_mlp_get_params(const rcp_mlp* m, float* params)
+{
+    return guarded([&]() {
+        if (!m || !params) throw rcp::ArgError("rcp: null argument");
+        m->m->get_params(params);
+        return RCP_OK;
+    });
+}
+
+int rcp_mlp_set_params(rcp_mlp* m, const float* params)
+{
+    return guarded([&]() {
+        if (!m || !params) throw rcp::ArgError("rcp: null argument");
+        m->m->set_params(params);
+        return RCP_OK;
+    });
+}
+
+int rcp_mlp_predict(rcp_mlp* m, const float* X, size_t B, float* Y)
+{
+    return guarded([&]() {
+        if (!m || (B && (!X || !Y))) throw rcp::ArgError("rcp: null argument");
+        m->m->predict(X, B, Y);
+        return RCP_OK;
+    });
+}
+
+int rcp_mlp_gradients(rcp_mlp* m, const float* X, const float* T, size_t B, float* loss,
+                      float* grad)
+{
+    return guarded([&]() {
+        if (!m || !X || !T || !loss || !grad) throw rcp::ArgError("rcp: null argument");
+        m->m->gradients(X, T, B, loss, grad);
+        return RCP_OK;
+    });
+}
+
+int rcp_mlp_train(rcp_mlp* m, const float* X, const float* T, size_t n, size_t batch,
+                  size_t epochs, float lr, uint64_t seed, float* losses)
+{
+    return guarded([&]() {
+        if (!m || !X || !T) throw rcp::ArgError("rcp: null argument");
+        m->m->train(X, T, n, batch, epochs, lr, seed, losses);
         return RCP_OK;
     });
 }

@@ -422,7 +422,20 @@ void DeviceState::diameters_ref(double kappa, std::vector<double>& D)
 
 // ------------------------------------------------------------- Engine
 
-Engine::Engine(const rcp_run_config& c)
+Engine::Engine(const rcp_run_config& c) { setup(c, false); }
+
+Engine::Engine(const rcp_run_config& c, Hosted)
+{
+    if (c.exact_reductions) throw ArgError("rcp: batched runs use the fast reductions");
+    if (c.fix_height || c.has_target_phi)
+        throw ArgError("rcp: fix_height and target_phi runs are not batched");
+    if (c.fixed)
+        for (std::size_t i = 0; i < c.N; ++i)
+            if (c.fixed[i]) throw ArgError("rcp: runs with frozen diameters are not batched");
+    setup(c, true);
+}
+
+void Engine::setup(const rcp_run_config& c, bool hosted)
 {
     if (c.N == 0) throw ArgError("rcp: N must be >= 1");
     if (c.ndim < 1 || c.ndim > (size_t)kMaxDim) throw ArgError("rcp: Ndim must be in 1..8");
@@ -502,8 +515,10 @@ Engine::Engine(const rcp_run_config& c)
     phi_ = current_volume / box_volume / phi_modifier_;
 
     // step cap (RG:4713-4737), speed preset "quick"
+    const bool deadline_set = c.rung1_deadline > 100 && c.rung1_deadline < 10000000;
     step_cap_ = c.max_steps > 0 ? c.max_steps : [&]() {
         double rung1 = 2.0 * 4000.0 + 9.0 * std::sqrt(double(N_) * double(nd_)) - 2000.0;
+        if (deadline_set) rung1 = static_cast<double>(c.rung1_deadline);
         if (rung1 < 1500.0) rung1 = 1500.0;
         const std::size_t formula = static_cast<std::size_t>(rung1 * double(2 + 3));
         return std::max<std::size_t>(60000, formula);
@@ -524,7 +539,9 @@ Engine::Engine(const rcp_run_config& c)
     if (nd_ >= 5) alpha_ = alpha_max_ = 0.0025;
     mu_ = c.has_mu ? c.mu : 2.5e-4;
     runtime_fix_ = c.fix_diameter != 0;
+    // rung-1 deadline (RG:5855-5885): the override, else the "quick" preset
     rung1_deadline_ = [&]() {
+        if (deadline_set) return c.rung1_deadline;
         double dl = 2.0 * 4000.0 + 9.0 * std::sqrt(double(N_) * double(nd_)) - 2000.0;
         if (dl < 1500.0) dl = 1500.0;
         return static_cast<std::size_t>(dl);
@@ -532,8 +549,6 @@ Engine::Engine(const rcp_run_config& c)
 
     // D0 caches before the initial reorder (RG:4808-4826)
     recompute_d0_cache(D, fixed);
-
-    // ---- device state
     geo_.N = (int)N_;
     geo_.ndim = (int)nd_;
     for (std::size_t d = 0; d < nd_; ++d) {
@@ -542,6 +557,15 @@ Engine::Engine(const rcp_run_config& c)
         geo_.walls[d] = walls_[d];
     }
     geo_.curved_T = curved_T_;
+    if (hosted) {
+        start_x_ = std::move(x);
+        start_D_ = std::move(D);
+        kappa_D_ = 1.0;
+        kappa_prev_ = 1.0;
+        return;
+    }
+
+    // ---- device state
     hip_check(hipStreamCreate(&stream_), "hipStreamCreate");
     if (opt_.time_kernels)
         for (auto& ev : ev_) hip_check(hipEventCreate(&ev), "hipEventCreate");
@@ -628,6 +652,15 @@ void Engine::recompute_d0_cache(const std::vector<double>& D0, const std::vector
     }
     const double advantage = (nd_ == 2 || nd_ == 3) ? 4.0 : 3.0;
     polydisperse_ = classes_ > 1 && single > advantage * multi;
+    // particles that can land in the wide classes (D > dmax 2^-wide at any
+    // binning; the margin absorbs the rounding of D0 kappa)
+    const int wide = std::max(0, classes_ - 2);
+    wide_bound_ = 0;
+    if (wide > 0) {
+        const double cut = d0max * std::ldexp(1.0, -wide) * (1.0 - 1e-9);
+        for (std::size_t i = 0; i < N_; ++i)
+            if (D0[i] > cut) ++wide_bound_;
+    }
 }
 
 double Engine::dmax_now() const
@@ -702,13 +735,12 @@ bool Engine::tiles_supported() const
     return opt_.neighbor_mode == 3 || (opt_.neighbor_mode == 0 && !polydisperse_);
 }
 
-void Engine::multi_rebin(double skin)
+void Engine::plan_multi_grid(double skin, MultiGrid& mg) const
 {
     // One grid per size class, cells at least as wide as the class's largest
     // diameter plus the skin (the step kernel sizes each particle's stencil
     // from its own reach, so the edge only sets the work per cell).
     const double dmax = dmax_now();
-    MultiGrid& mg = mg_;
     mg.C = classes_;
     long long base = 0;
     for (int c = 0; c < mg.C; ++c) {
@@ -725,18 +757,97 @@ void Engine::multi_rebin(double skin)
     // classes at least four times the smallest can touch many small
     // particles: they take a wavefront each
     mg.wide = std::max(0, mg.C - 2);
+}
+
+void Engine::note_binning(double skin)
+{
+    tile_skin_ = skin;
+    tile_binned_ = true;
+    tile_invalid_ = false;
+    kappa_b_ = kappa_D_;
+    fh_drift_ = 0.0;
+    need_build_ = true;   // list rows (if any) are gone with the re-binning
+    ++tile_rebins_;
+}
+
+void Engine::multi_rebin(double skin)
+{
+    plan_multi_grid(skin, mg_);
     begin_timer(0);
-    st_->rebin_multi(geo_, mg, kappa_D_);
+    st_->rebin_multi(geo_, mg_, kappa_D_);
     if (skin > 0.0)
         hip_check(hipMemcpyAsync(st_->anchor, st_->x, N_ * nd_ * sizeof(double),
                                  hipMemcpyDeviceToDevice, stream_), "anchor copy");
     end_timer(0);
-    tile_skin_ = skin;
-    tile_binned_ = true;
-    kappa_b_ = kappa_D_;
-    fh_drift_ = 0.0;
-    need_build_ = true;
-    ++tile_rebins_;
+    note_binning(skin);
+}
+
+bool Engine::binning_due(double growth_step, double& skin) const
+{
+    // the binning holds while no particle can have crossed half the skin
+    const double growth = std::max(0.0, D0max_free_ * (kappa_D_ - kappa_b_));
+    const double reach = 2.0 * std::sqrt(maxdisp2_) + growth + 2.0 * fh_drift_;
+    const bool stale = tile_skin_ <= 0.0 || !(reach < tile_skin_ * (1.0 - 1e-6));
+    if (tile_binned_ && !stale && !tile_invalid_) return false;
+    // a skin pays off once a binning lives two steps or more
+    const double per_step = 2.0 * std::sqrt(maxstep2_) + growth_step + 2.0 * fh_step_;
+    const double shell = shell0_ * kappa_D_;
+    skin = per_step * 2.0 < shell ? shell : 0.0;
+    return true;
+}
+
+// ------------------------------------------------------ hosted controllers
+
+long long Engine::cell_budget() const
+{
+    long long total = 0;
+    for (int c = 0; c < classes_; ++c)
+        total += std::max<long long>(64, 2 * (c < (int)class_count_.size() ? class_count_[c] : 1));
+    return total;
+}
+
+void Engine::hosted_initial(MultiGrid& mg)
+{
+    kappa_D_ = 1.0;
+    kappa_prev_ = 1.0;
+    plan_multi_grid(0.0, mg);
+    note_binning(0.0);
+}
+
+bool Engine::hosted_binning(MultiGrid& mg)
+{
+    const double growth_step = D0max_free_ * std::fabs(kappa_D_ - kappa_prev_);
+    kappa_prev_ = kappa_D_;
+    double skin = 0.0;
+    if (!binning_due(growth_step, skin)) return false;
+    plan_multi_grid(skin, mg);
+    note_binning(skin);
+    return true;
+}
+
+void Engine::hosted_launch(HostedLaunch& l) const
+{
+    l.kappa = kappa_D_;
+    l.alpha = alpha_;
+    l.pad = 0.5 * tile_skin_;
+    l.grow = std::max(1.0, kappa_D_ / kappa_b_);
+    l.anchor = tile_skin_ > 0.0;
+}
+
+void Engine::hosted_complete(const Scalars& s, const StepPlan& p)
+{
+    if (s.v[S_OVERFLOW] != 0.0) ++overflow_steps_;
+    ++tile_steps_;
+    complete_step(s, p);
+    if (hist_filled_ >= step_cap_) done_ = true;
+}
+
+void Engine::set_final_state(std::vector<double> x, std::vector<double> D,
+                             std::vector<uint64_t> origin)
+{
+    fin_x_ = std::move(x);
+    fin_D_ = std::move(D);
+    fin_origin_ = std::move(origin);
 }
 
 void Engine::tile_rebin(double skin)
@@ -752,12 +863,7 @@ void Engine::tile_rebin(double skin)
         hip_check(hipMemcpyAsync(st_->anchor, st_->x, N_ * nd_ * sizeof(double),
                                  hipMemcpyDeviceToDevice, stream_), "anchor copy");
     end_timer(0);
-    tile_skin_ = skin;
-    tile_binned_ = true;
-    kappa_b_ = kappa_D_;
-    fh_drift_ = 0.0;
-    need_build_ = true;   // list rows (if any) are gone with the re-binning
-    ++tile_rebins_;
+    note_binning(skin);
 }
 
 // --------------------------------------------------------------- device step
@@ -1033,7 +1139,7 @@ void Engine::release_cascade()
     tile_invalid_ = true;
 }
 
-bool Engine::step_once()
+bool Engine::plan_step(StepPlan& p)
 {
     const std::size_t step = hist_filled_ + 1;
     steps_var_ = step;
@@ -1042,13 +1148,10 @@ bool Engine::step_once()
         done_ = true;
         return false;
     }
-    // periodic spatial reorder (RG:5661-5663): a new reference order; the
-    // rows are ordered by tag, so they are rebuilt
-    if (opt_.reorder_interval > 0 && step > 1 && (step % opt_.reorder_interval == 0)) {
-        st_->reference_reorder(geo_);
-        ++reorders_;
-        need_build_ = true;
-    }
+    // periodic spatial reorder (RG:5661-5663): done by the caller between
+    // this and the device step (it only relabels particles)
+    p.step = step;
+    p.reorder = opt_.reorder_interval > 0 && step > 1 && (step % opt_.reorder_interval == 0);
     schedule_alpha(step);
 
     // growth (RG:5723-5834): D = D0 * kappa evaluated inside the kernels
@@ -1073,7 +1176,8 @@ bool Engine::step_once()
     phi_h_[step - 1] = phi_ / phi_modifier_;
 
     // rung 1 -> 0 (RG:5840-5931)
-    bool release_now = false;
+    bool& release_now = p.release_now;
+    release_now = false;
     if (step > 500 && mu_flag_ == 1) {
         if (step > static_cast<std::size_t>(static_cast<int>(rung1_deadline_))) {
             mu_decay_target_ = mu_ / 10.0;
@@ -1086,6 +1190,49 @@ bool Engine::step_once()
         }
     }
 
+
+    // forces (RG:6211-6237) + fused ADAM (RG:6409-6521)
+    p.mu_eff = (mu_flag_ == 1)
+        ? mu_ * (1.0 + (5.0 - 1.0) * std::exp(-static_cast<double>(step) / 350.0)) : mu_;
+    p.inv_b1 = 1.0 / (1.0 - std::pow(0.9, t_));
+    p.inv_b2 = 1.0 / (1.0 - std::pow(0.999, t_));
+    return true;
+}
+
+void Engine::complete_step(const Scalars& s, const StepPlan& p)
+{
+    const std::size_t step = p.step;
+    finalize_scalars(s, p.mu_eff);
+    force_magnitude_ = mean_force_ / Fmean_;
+    F_h_[step - 1] = force_magnitude_;
+    ov_h_[step - 1] = max_min_dist_;
+    mu_h_[step - 1] = mu_;
+    al_h_[step - 1] = alpha_;
+    fl_h_[step - 1] = static_cast<int8_t>(mu_flag_);
+
+    // kappa ADAM (RG:6354-6358) and step (RG:6405-6407)
+    m_kappa_ = 0.9 * m_kappa_ - (1 - 0.9) * dkappa_;
+    v_kappa_ = 0.999 * v_kappa_ + (1 - 0.999) * (dkappa_ * dkappa_);
+    const double m_hat = m_kappa_ / (1.0 - std::pow(0.9, t_));
+    const double v_hat = v_kappa_ / (1.0 - std::pow(0.999, t_));
+    if (!runtime_fix_) kappa_ = kappa_ - alpha_ * m_hat / (std::sqrt(v_hat) + 1e-8);
+
+    if (p.release_now) release_cascade();
+    U_h_[step - 1] = U_;
+    hist_filled_ = step;
+}
+
+bool Engine::step_once()
+{
+    StepPlan p;
+    if (!plan_step(p)) return false;
+    if (p.reorder) {
+        // a new reference order; the rows are ordered by tag, so they are
+        // rebuilt
+        st_->reference_reorder(geo_);
+        ++reorders_;
+        need_build_ = true;
+    }
     // Neighbour handling.  List mode reuses the rows until a particle could
     // have crossed half the skin (diameter growth and box stretch included).
     // When rows live only a couple of steps (violent early phase) the direct
@@ -1096,18 +1243,10 @@ bool Engine::step_once()
     const bool multi = multi_supported();
     if (multi || tiles_supported()) {
         mode_ = multi ? Mode::Multi : Mode::Tile;
-        // the binning holds while no particle can have crossed half the skin
-        const double growth = std::max(0.0, D0max_free_ * (kappa_D_ - kappa_b_));
-        const double reach = 2.0 * std::sqrt(maxdisp2_) + growth + 2.0 * fh_drift_;
-        const bool stale = tile_skin_ <= 0.0 || !(reach < tile_skin_ * (1.0 - 1e-6));
-        if (!tile_binned_ || stale || tile_invalid_) {
-            // a skin pays off once a binning lives two steps or more
-            const double per_step = 2.0 * std::sqrt(maxstep2_) + growth_step + 2.0 * fh_step_;
-            const double shell = shell0_ * kappa_D_;
-            const double skin = per_step * 2.0 < shell ? shell : 0.0;
+        double skin = 0.0;
+        if (binning_due(growth_step, skin)) {
             if (multi) multi_rebin(skin);
             else tile_rebin(skin);
-            tile_invalid_ = false;
         }
     } else if (opt_.neighbor_mode == 2) {
         mode_ = Mode::Direct;
@@ -1147,47 +1286,37 @@ bool Engine::step_once()
         ++list_age_;
     }
 
-    // forces (RG:6211-6237) + fused ADAM (RG:6409-6521)
-    const double mu_eff = (mu_flag_ == 1)
-        ? mu_ * (1.0 + (5.0 - 1.0) * std::exp(-static_cast<double>(step) / 350.0)) : mu_;
-    const double inv_b1 = 1.0 / (1.0 - std::pow(0.9, t_));
-    const double inv_b2 = 1.0 / (1.0 - std::pow(0.999, t_));
     // In the release step the reference clears the mask before the force call
     // while D still holds the frozen sizes: every particle counts as active.
     const double* rk = nullptr;
-    if (release_now) {
+    if (p.release_now) {
         if (!st_->rk) st_->rk = st_->alloc<double>(N_);
         if (launch_rk_active(st_->D0, st_->fixed, kappa_D_, st_->rk, (int)N_, stream_))
             throw HipFailure("rk launch");
         rk = st_->rk;
     }
-    const Scalars s = step_device(true, alpha_, inv_b1, inv_b2, has_fixed_ ? 1 : 0, rk, mode_);
-    finalize_scalars(s, mu_eff);
-    force_magnitude_ = mean_force_ / Fmean_;
-    F_h_[step - 1] = force_magnitude_;
-    ov_h_[step - 1] = max_min_dist_;
-    mu_h_[step - 1] = mu_;
-    al_h_[step - 1] = alpha_;
-    fl_h_[step - 1] = static_cast<int8_t>(mu_flag_);
-
-    // kappa ADAM (RG:6354-6358) and step (RG:6405-6407)
-    m_kappa_ = 0.9 * m_kappa_ - (1 - 0.9) * dkappa_;
-    v_kappa_ = 0.999 * v_kappa_ + (1 - 0.999) * (dkappa_ * dkappa_);
-    const double m_hat = m_kappa_ / (1.0 - std::pow(0.9, t_));
-    const double v_hat = v_kappa_ / (1.0 - std::pow(0.999, t_));
-    if (!runtime_fix_) kappa_ = kappa_ - alpha_ * m_hat / (std::sqrt(v_hat) + 1e-8);
-
-    if (release_now) release_cascade();
-    U_h_[step - 1] = U_;
-    hist_filled_ = step;
+    const Scalars s = step_device(true, alpha_, p.inv_b1, p.inv_b2, has_fixed_ ? 1 : 0, rk, mode_);
+    complete_step(s, p);
     return true;
 }
 
-void Engine::current_diameters(std::vector<double>& D) { st_->diameters_ref(kappa_D_, D); }
+void Engine::current_diameters(std::vector<double>& D)
+{
+    if (st_) st_->diameters_ref(kappa_D_, D);
+    else D = fin_D_;
+}
 
-void Engine::current_positions(std::vector<double>& x) { st_->to_ref(st_->x, (int)nd_, x); }
+void Engine::current_positions(std::vector<double>& x)
+{
+    if (st_) st_->to_ref(st_->x, (int)nd_, x);
+    else x = fin_x_;
+}
 
-void Engine::origin(std::vector<uint64_t>& o) { st_->origin_ref(o); }
+void Engine::origin(std::vector<uint64_t>& o)
+{
+    if (st_) st_->origin_ref(o);
+    else o = fin_origin_;
+}
 
 void Engine::histories(double* phi, double* F, double* U, double* ov, double* mu, double* al,
                        int8_t* fl) const

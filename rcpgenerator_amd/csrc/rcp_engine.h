@@ -122,7 +122,56 @@ private:
 
 class Engine {
 public:
+    struct Scalars { double v[S_NUM]; };
+    // host decisions of one step, taken before its force evaluation
+    struct StepPlan {
+        std::size_t step = 0;
+        bool reorder = false, release_now = false;
+        double mu_eff = 0.0, inv_b1 = 1.0, inv_b2 = 1.0;
+    };
+
     explicit Engine(const rcp_run_config& c);
+    // Controller only (a member of a batch whose device state lives
+    // elsewhere): set-up and the schedule, no device work.  Runs that need
+    // device access from the controller (a frozen mask, fix_height,
+    // target_phi) or exact reductions are rejected.
+    struct Hosted {};
+    Engine(const rcp_run_config& c, Hosted);
+
+    // One step split at its device work (step_once = plan_step, device
+    // work, complete_step).  plan_step returns false once the run is done.
+    bool plan_step(StepPlan& p);
+    void complete_step(const Scalars& s, const StepPlan& p);
+    // first force evaluation of a hosted controller (standard weights, mu)
+    void complete_initial(const Scalars& s) { finalize_scalars(s, mu_); }
+    // hosted set-up outputs: start positions and diameters after the
+    // start-phi rescale, reference order
+    const std::vector<double>& start_positions() const { return start_x_; }
+    const std::vector<double>& start_diameters() const { return start_D_; }
+    std::size_t step_cap() const { return step_cap_; }
+    int reorder_interval() const { return (int)opt_.reorder_interval; }
+    const Geometry& geometry() const { return geo_; }
+    void count_reorder() { ++reorders_; }
+    // Size-class neighbour handling of a hosted controller, decided as
+    // step_once decides it in size-class mode.  hosted_initial: the binning
+    // of the first force evaluation; hosted_binning: true when this step
+    // re-bins (mg: the new grids).
+    void hosted_initial(MultiGrid& mg);
+    bool hosted_binning(MultiGrid& mg);
+    // wide-class particles at most (size the one-wave step's workgroups)
+    long long wide_bound() const { return wide_bound_; }
+    // cells of a size-class binning at most
+    long long cell_budget() const;
+    struct HostedLaunch {
+        double kappa = 1.0, alpha = 0.0, pad = 0.0, grow = 1.0;
+        bool anchor = false;   // the step measures drift from the anchor copy
+    };
+    void hosted_launch(HostedLaunch& l) const;
+    // complete_step plus the step counters; marks the run done at the cap
+    void hosted_complete(const Scalars& s, const StepPlan& p);
+    // final positions, diameters, origin (reference order) of a hosted run
+    void set_final_state(std::vector<double> x, std::vector<double> D,
+                         std::vector<uint64_t> origin);
     ~Engine();
     Engine(const Engine&) = delete;
     Engine& operator=(const Engine&) = delete;
@@ -165,7 +214,7 @@ public:
     }
 
 private:
-    struct Scalars { double v[S_NUM]; };
+    void setup(const rcp_run_config& c, bool hosted);
     enum class Osc { Inactive, Active, Final, Done };
     enum class Mode { List, Direct, Tile, Multi };
 
@@ -176,6 +225,9 @@ private:
                         const double* rk, Mode mode);
     void tile_rebin(double skin);
     void multi_rebin(double skin);
+    void plan_multi_grid(double skin, MultiGrid& mg) const;
+    void note_binning(double skin);
+    bool binning_due(double growth_step, double& skin) const;
     bool tiles_supported() const;
     bool multi_supported() const;
     void finalize_scalars(const Scalars& s, double mu_eff);
@@ -226,6 +278,10 @@ private:
     std::size_t hist_filled_ = 0, steps_var_ = 0;
     bool done_ = false, initial_recorded_ = false;
     std::vector<TraceSample> trace_;
+    std::vector<double> start_x_, start_D_;   // hosted controllers only
+    std::vector<double> fin_x_, fin_D_;
+    std::vector<uint64_t> fin_origin_;
+    long long wide_bound_ = 0;
 
     // neighbour handling
     Mode mode_ = Mode::List;

@@ -175,6 +175,48 @@ size_t multi_partial_rows(size_t n);
 // inv_tag[tag[w]] = w
 int launch_inv_tag(const uint32_t* tag, uint32_t* inv_tag, int N, hipStream_t s);
 
+// ---------------------------------------------------------- batched runs
+// Packings of one dimension step together: each launch covers the
+// particles (or cells) of a set of packings, every workgroup working for
+// one packing, found by binary search over a prefix of per-packing
+// workgroup counts.  Per packing the arithmetic, the partial rows and
+// their fold are those of launch_rebin_multi / launch_step_multi on that
+// packing alone, so a batched run reproduces its single run bit for bit.
+struct BatchPack {
+    StepArgs a;          // pointers at this packing's arrays; partials at its
+                         // first row; tile_len = workgroups of its per-thread step
+    MultiGrid mg;
+    int wide_blocks = 0; // workgroups of its one-wave-per-particle step
+    int rows = 0;        // partial rows (tile_len + wide_blocks)
+};
+struct BatchBin {
+    BinArgs b;           // b.grid.total = cells of the size-class table
+    MultiGrid mg;
+    double* anchor_o = nullptr;   // second copy of the new-order positions, or null
+};
+// Workgroup table: entry e owns workgroups prefix[e] .. prefix[e+1]-1 and
+// works for item ids[e] (ids null: item e).  Device pointers.
+struct BatchTable {
+    const unsigned* prefix = nullptr;
+    const int* ids = nullptr;
+    int n = 0;
+    unsigned blocks = 0;          // prefix[n]
+};
+constexpr int kBatchStepBlock = 128;    // particles per workgroup of the per-thread step
+constexpr int kBatchBinBlock = 256;     // particles / cells per workgroup of the binning
+constexpr int kBatchWideGrid = 1024;    // most workgroups of one packing's wide step
+constexpr int kBatchWideWaves = 4;      // particles per wide workgroup and pass
+// Step of the packings in packs[0..npacks): per-thread kernel over `multi`,
+// one-wave kernel over `wide` (entries with wide classes), then one fold
+// workgroup per packing writing results[p * S_NUM ...].
+int launch_batch_step(int ndim, const BatchPack* packs, int npacks, const BatchTable& multi,
+                      const BatchTable& wide, double* results, hipStream_t s,
+                      hipEvent_t mark = nullptr);
+// Re-binning of the packings in bins[0..nbins): `parts` tiles their
+// particles, `cells` their cell tables (total + 1 entries each).
+int launch_batch_rebin(int ndim, const BatchBin* bins, int nbins, const BatchTable& parts,
+                       const BatchTable& cells, hipStream_t s);
+
 // ---------------------------------------------------------------- lists
 struct ListArgs {
     Geometry g;

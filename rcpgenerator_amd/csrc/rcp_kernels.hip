@@ -135,8 +135,8 @@ __device__ __forceinline__ void finish_grid(const StepArgs& a, double (&blk)[S_N
 
 constexpr int kFoldBlock = 1024;
 
-__global__ __launch_bounds__(kFoldBlock) void k_fold(const double* partials, unsigned rows,
-                                                     double* result)
+// Rows b = t, t + kFoldBlock, ... go to thread t; then one block reduction.
+__device__ __forceinline__ void fold_rows(const double* partials, unsigned rows, double* result)
 {
     double v[S_NUM];
 #pragma unroll
@@ -156,6 +156,12 @@ __global__ __launch_bounds__(kFoldBlock) void k_fold(const double* partials, uns
 #pragma unroll
         for (int k = 0; k < S_NUM; ++k) result[k] = tot[k];
     }
+}
+
+__global__ __launch_bounds__(kFoldBlock) void k_fold(const double* partials, unsigned rows,
+                                                     double* result)
+{
+    fold_rows(partials, rows, result);
 }
 
 // ------------------------------------------------------ per-particle parts
@@ -1001,15 +1007,13 @@ __device__ __forceinline__ void for_each_class_candidate(const StepArgs& a, cons
 constexpr int kMultiBlock = 128;
 constexpr int kMultiCap = 24;
 
+// Particle q of the per-thread size-class step (thread tid of its
+// workgroup, buf its contact buffer): forces, pair and wall terms into acc.
 template <int ND>
-__global__ __launch_bounds__(kMultiBlock) void k_step_multi(StepArgs a, MultiGrid mg)
+__device__ __forceinline__ void multi_particle(const StepArgs& a, const MultiGrid& mg, int q,
+                                               int tid, uint32_t (&buf)[kMultiCap][kMultiBlock],
+                                               double (&acc)[S_NUM])
 {
-    __shared__ uint32_t buf[kMultiCap][kMultiBlock];
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    const int tid = threadIdx.x;
-    double acc[S_NUM];
-#pragma unroll
-    for (int k = 0; k < S_NUM; ++k) acc[k] = 0.0;
     // the largest classes lead the working order; k_step_wide takes them
     const uint32_t n_wide = mg.wide > 0 ? a.cell_start[mg.base[mg.wide]] : 0u;
     if (q < a.g.N && (uint32_t)q >= n_wide) {
@@ -1061,6 +1065,16 @@ __global__ __launch_bounds__(kMultiBlock) void k_step_multi(StepArgs a, MultiGri
         }
         particle_tail<ND>(a, q, o, F, acc);
     }
+}
+
+template <int ND>
+__global__ __launch_bounds__(kMultiBlock) void k_step_multi(StepArgs a, MultiGrid mg)
+{
+    __shared__ uint32_t buf[kMultiCap][kMultiBlock];
+    double acc[S_NUM];
+#pragma unroll
+    for (int k = 0; k < S_NUM; ++k) acc[k] = 0.0;
+    multi_particle<ND>(a, mg, blockIdx.x * blockDim.x + threadIdx.x, threadIdx.x, buf, acc);
     double blk[S_NUM];
     block_reduce<kMultiBlock>(acc, blk);
     finish_grid<kMultiBlock>(a, blk);
@@ -1088,20 +1102,30 @@ constexpr int kWideCap = 1024;                 // contacts buffered per particle
 constexpr int kWideBlock = kWideWaves * kWave;
 
 template <int ND>
-__global__ __launch_bounds__(kWideBlock) void k_step_wide(StepArgs a, MultiGrid mg)
+struct WideLds {
+    unsigned long long keys[kWideWaves][kWideCap];
+    double prod[kWideWaves][ND][kWave];
+    double terms[kWideWaves][4][kWave];
+    int flag[kWideWaves][kWave];
+    int cnt[kWideWaves];
+};
+
+// The particles of workgroup blk of the one-wave-per-particle step: wave w
+// = blk * kWideWaves + wv of kWideGrid * kWideWaves takes w, w + waves, ...
+template <int ND>
+__device__ __forceinline__ void wide_particles(const StepArgs& a, const MultiGrid& mg,
+                                               unsigned blk, WideLds<ND>& L,
+                                               double (&acc)[S_NUM])
 {
-    __shared__ unsigned long long keys[kWideWaves][kWideCap];
-    __shared__ double prod[kWideWaves][ND][kWave];
-    __shared__ double terms[kWideWaves][4][kWave];
-    __shared__ int flag[kWideWaves][kWave];
-    __shared__ int cnt[kWideWaves];
+    auto& keys = L.keys;
+    auto& prod = L.prod;
+    auto& terms = L.terms;
+    auto& flag = L.flag;
+    auto& cnt = L.cnt;
     const int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
-    double acc[S_NUM];
-#pragma unroll
-    for (int k = 0; k < S_NUM; ++k) acc[k] = 0.0;
     const uint32_t n_wide = a.cell_start[mg.base[mg.wide]];
-    const unsigned waves = gridDim.x * kWideWaves;
-    for (uint32_t q = blockIdx.x * kWideWaves + wv; q < n_wide; q += waves) {
+    const unsigned waves = (unsigned)kWideGrid * kWideWaves;
+    for (uint32_t q = blk * kWideWaves + wv; q < n_wide; q += waves) {
         Own<ND> o;
         load_own<ND>(a, (int)q, o);
         if (lane == 0) cnt[wv] = 0;
@@ -1244,15 +1268,33 @@ __global__ __launch_bounds__(kWideBlock) void k_step_wide(StepArgs a, MultiGrid 
         if (lane == 0) particle_tail<ND>(a, (int)q, o, F, acc);
         wave_sync();
     }
-    double blk[S_NUM];
-    block_reduce<kWideBlock>(acc, blk);
+}
+
+// Block totals of workgroup blk go to partial row a.tile_len + blk, after
+// the per-thread kernel's rows.
+template <int BLOCK>
+__device__ __forceinline__ void store_row(const StepArgs& a, unsigned blk, double (&acc)[S_NUM])
+{
+    double tot[S_NUM];
+    block_reduce<BLOCK>(acc, tot);
     if (threadIdx.x == 0) {
-        // rows after the per-thread kernel's rows
-        const size_t row = (size_t)a.tile_len + blockIdx.x;
+        const size_t row = (size_t)a.tile_len + blk;
 #pragma unroll
-        for (int k = 0; k < S_NUM; ++k) a.partials[row * S_NUM + k] = blk[k];
+        for (int k = 0; k < S_NUM; ++k) a.partials[row * S_NUM + k] = tot[k];
     }
 }
+
+template <int ND>
+__global__ __launch_bounds__(kWideBlock) void k_step_wide(StepArgs a, MultiGrid mg)
+{
+    __shared__ WideLds<ND> L;
+    double acc[S_NUM];
+#pragma unroll
+    for (int k = 0; k < S_NUM; ++k) acc[k] = 0.0;
+    wide_particles<ND>(a, mg, blockIdx.x, L, acc);
+    store_row<kWideBlock>(a, blockIdx.x, acc);
+}
+
 
 // -------------------------------------------------------- exact reduction
 //
@@ -1433,10 +1475,8 @@ __global__ __launch_bounds__(kBlock) void k_bin(BinArgs a)
 }
 
 template <int ND>
-__global__ __launch_bounds__(kBlock) void k_bin_multi(BinArgs a, MultiGrid mg)
+__device__ __forceinline__ void bin_multi_one(const BinArgs& a, const MultiGrid& mg, int w)
 {
-    const int w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= a.g.N) return;
     const bool f = a.fixed && a.fixed[w];
     const double D = f ? a.D0[w] : a.D0[w] * a.kappa;
     int c = 0;
@@ -1454,21 +1494,30 @@ __global__ __launch_bounds__(kBlock) void k_bin_multi(BinArgs a, MultiGrid mg)
     a.slot[w] = atomicAdd(&a.cell_count[cell], 1u);
 }
 
+template <int ND>
+__global__ __launch_bounds__(kBlock) void k_bin_multi(BinArgs a, MultiGrid mg)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < a.g.N) bin_multi_one<ND>(a, mg, w);
+}
+
+__device__ __forceinline__ void place_one(const BinArgs& a, int w)
+{
+    a.perm[a.cell_start[a.cell_of[w]] + a.slot[w]] = (uint32_t)w;
+}
+
 __global__ __launch_bounds__(kBlock) void k_place(BinArgs a)
 {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= a.g.N) return;
-    a.perm[a.cell_start[a.cell_of[w]] + a.slot[w]] = (uint32_t)w;
+    if (w < a.g.N) place_one(a, w);
 }
 
 // One thread per cell: order the cell's members by their previous working
 // position, which makes the new order independent of atomic arrival order.
 // Cells of up to 16 members are sorted in registers by a bitonic network
 // (padding sorts last); larger ones by insertion in memory.
-__global__ __launch_bounds__(kBlock) void k_cell_order(BinArgs a)
+__device__ __forceinline__ void cell_order_one(const BinArgs& a, long long c)
 {
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= a.grid.total) return;
     const uint32_t q0 = a.cell_start[c], q1 = a.cell_start[c + 1];
     const uint32_t n = q1 - q0;
     if (n <= 1) return;
@@ -1510,21 +1559,36 @@ __global__ __launch_bounds__(kBlock) void k_cell_order(BinArgs a)
     }
 }
 
-template <int ND>
-__global__ __launch_bounds__(kBlock) void k_permute(BinArgs a)
+__global__ __launch_bounds__(kBlock) void k_cell_order(BinArgs a)
 {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= a.g.N) return;
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < a.grid.total) cell_order_one(a, c);
+}
+
+// New position q takes old position perm[q]; anchor_o (optional) receives
+// a second copy of the positions.
+template <int ND>
+__device__ __forceinline__ void permute_one(const BinArgs& a, int q, double* anchor_o)
+{
     const uint32_t w = a.perm[q];
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
-        a.x_o[(size_t)q * ND + d] = a.x[(size_t)w * ND + d];
+        const double xv = a.x[(size_t)w * ND + d];
+        a.x_o[(size_t)q * ND + d] = xv;
+        if (anchor_o) anchor_o[(size_t)q * ND + d] = xv;
         a.m_o[(size_t)q * ND + d] = a.m[(size_t)w * ND + d];
         a.v_o[(size_t)q * ND + d] = a.v[(size_t)w * ND + d];
     }
     a.D0_o[q] = a.D0[w];
     a.tag_o[q] = a.tag[w];
     if (a.fixed) a.fixed_o[q] = a.fixed[w];
+}
+
+template <int ND>
+__global__ __launch_bounds__(kBlock) void k_permute(BinArgs a)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < a.g.N) permute_one<ND>(a, q, nullptr);
 }
 
 // ------------------------------------------------------------------ lists
@@ -1960,6 +2024,133 @@ __global__ void k_rk_active(const double* D0, const int8_t* fixed, double kappa,
     if (k < N) rk[k] = ((fixed && fixed[k]) ? D0[k] : D0[k] * kappa) * 0.5;
 }
 
+// ------------------------------------------------------------ batched runs
+//
+// Each workgroup finds its packing in a table of per-packing workgroup
+// ranges and runs the single-packing code on that packing's arrays, with the
+// same particle-to-workgroup assignment and the same partial rows.
+
+static_assert(kBatchStepBlock == kMultiBlock, "batched step tiles as the single step");
+static_assert(kBatchWideGrid == kWideGrid && kBatchWideWaves == kWideWaves,
+              "batched wide step assigns particles as the single step");
+
+// the largest e with prefix[e] <= blk
+__device__ __forceinline__ int table_entry(const BatchTable& t, unsigned blk)
+{
+    int lo = 0, hi = t.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (t.prefix[mid] <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int table_item(const BatchTable& t, int e)
+{
+    return t.ids ? t.ids[e] : e;
+}
+
+template <int ND>
+__global__ __launch_bounds__(kMultiBlock) void kb_step_multi(const BatchPack* __restrict__ packs,
+                                                            BatchTable t)
+{
+    __shared__ uint32_t buf[kMultiCap][kMultiBlock];
+    const int e = table_entry(t, blockIdx.x);
+    const BatchPack& P = packs[table_item(t, e)];
+    const unsigned lb = blockIdx.x - t.prefix[e];
+    double acc[S_NUM];
+#pragma unroll
+    for (int k = 0; k < S_NUM; ++k) acc[k] = 0.0;
+    multi_particle<ND>(P.a, P.mg, (int)(lb * kMultiBlock + threadIdx.x), threadIdx.x, buf, acc);
+    double blk[S_NUM];
+    block_reduce<kMultiBlock>(acc, blk);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < S_NUM; ++k) P.a.partials[(size_t)lb * S_NUM + k] = blk[k];
+    }
+}
+
+template <int ND>
+__global__ __launch_bounds__(kWideBlock) void kb_step_wide(const BatchPack* __restrict__ packs,
+                                                          BatchTable t)
+{
+    __shared__ WideLds<ND> L;
+    const int e = table_entry(t, blockIdx.x);
+    const BatchPack& P = packs[table_item(t, e)];
+    const unsigned vb = blockIdx.x - t.prefix[e];
+    double acc[S_NUM];
+#pragma unroll
+    for (int k = 0; k < S_NUM; ++k) acc[k] = 0.0;
+    wide_particles<ND>(P.a, P.mg, vb, L, acc);
+    store_row<kWideBlock>(P.a, vb, acc);
+}
+
+__global__ __launch_bounds__(kFoldBlock) void kb_fold(const BatchPack* __restrict__ packs,
+                                                      double* results)
+{
+    const BatchPack& P = packs[blockIdx.x];
+    fold_rows(P.a.partials, (unsigned)P.rows, results + (size_t)blockIdx.x * S_NUM);
+}
+
+__global__ __launch_bounds__(kBatchBinBlock) void kb_clear_cells(const BatchBin* __restrict__ bins,
+                                                                 BatchTable t)
+{
+    const int e = table_entry(t, blockIdx.x);
+    const BatchBin& B = bins[table_item(t, e)];
+    const long long c = (long long)(blockIdx.x - t.prefix[e]) * kBatchBinBlock + threadIdx.x;
+    if (c <= B.b.grid.total) B.b.cell_count[c] = 0u;
+}
+
+template <int ND>
+__global__ __launch_bounds__(kBatchBinBlock) void kb_bin_multi(const BatchBin* __restrict__ bins,
+                                                               BatchTable t)
+{
+    const int e = table_entry(t, blockIdx.x);
+    const BatchBin& B = bins[table_item(t, e)];
+    const int w = (int)((blockIdx.x - t.prefix[e]) * kBatchBinBlock + threadIdx.x);
+    if (w < B.b.g.N) bin_multi_one<ND>(B.b, B.mg, w);
+}
+
+// one workgroup per packing: cell_start = exclusive scan of cell_count,
+// cell_start[total] = N
+__global__ __launch_bounds__(kFoldBlock) void kb_scan_cells(const BatchBin* __restrict__ bins)
+{
+    const BatchBin& B = bins[blockIdx.x];
+    const int n = (int)B.b.grid.total;
+    for (int c = threadIdx.x; c < n; c += kFoldBlock) B.b.cell_start[c] = B.b.cell_count[c];
+    __syncthreads();
+    block_scan_inplace<kFoldBlock>(B.b.cell_start, n);
+}
+
+__global__ __launch_bounds__(kBatchBinBlock) void kb_place(const BatchBin* __restrict__ bins,
+                                                           BatchTable t)
+{
+    const int e = table_entry(t, blockIdx.x);
+    const BatchBin& B = bins[table_item(t, e)];
+    const int w = (int)((blockIdx.x - t.prefix[e]) * kBatchBinBlock + threadIdx.x);
+    if (w < B.b.g.N) place_one(B.b, w);
+}
+
+__global__ __launch_bounds__(kBatchBinBlock) void kb_cell_order(const BatchBin* __restrict__ bins,
+                                                                BatchTable t)
+{
+    const int e = table_entry(t, blockIdx.x);
+    const BatchBin& B = bins[table_item(t, e)];
+    const long long c = (long long)(blockIdx.x - t.prefix[e]) * kBatchBinBlock + threadIdx.x;
+    if (c < B.b.grid.total) cell_order_one(B.b, c);
+}
+
+template <int ND>
+__global__ __launch_bounds__(kBatchBinBlock) void kb_permute(const BatchBin* __restrict__ bins,
+                                                             BatchTable t)
+{
+    const int e = table_entry(t, blockIdx.x);
+    const BatchBin& B = bins[table_item(t, e)];
+    const int q = (int)((blockIdx.x - t.prefix[e]) * kBatchBinBlock + threadIdx.x);
+    if (q < B.b.g.N) permute_one<ND>(B.b, q, B.anchor_o);
+}
+
 inline unsigned grid_for(size_t n, int block = kBlock)
 {
     return (unsigned)((n + block - 1) / block);
@@ -2391,6 +2582,34 @@ int launch_mean_force(const StepArgs& a, const double* F, int exact, hipStream_t
     k_mean_force<<<grid_for(a.g.N), kBlock, 0, s>>>(a, F);
     k_fold<<<1, kFoldBlock, 0, s>>>(a.partials, grid_for(a.g.N), a.result);
     if (exact) k_mean_force_exact<<<1, kWave, 0, s>>>(a);
+    return check_launch();
+}
+
+// ------------------------------------------------------ batched launches
+
+int launch_batch_step(int ndim, const BatchPack* packs, int npacks, const BatchTable& multi,
+                      const BatchTable& wide, double* results, hipStream_t s, hipEvent_t mark)
+{
+    if (npacks <= 0) return 0;
+    if (multi.blocks > 0)
+        RCP_DISPATCH_ND(ndim, kb_step_multi, <<<multi.blocks, kMultiBlock, 0, s>>>(packs, multi));
+    if (wide.blocks > 0)
+        RCP_DISPATCH_ND(ndim, kb_step_wide, <<<wide.blocks, kWideBlock, 0, s>>>(packs, wide));
+    if (mark && hipEventRecord(mark, s) != hipSuccess) return 1;
+    kb_fold<<<(unsigned)npacks, kFoldBlock, 0, s>>>(packs, results);
+    return check_launch();
+}
+
+int launch_batch_rebin(int ndim, const BatchBin* bins, int nbins, const BatchTable& parts,
+                       const BatchTable& cells, hipStream_t s)
+{
+    if (nbins <= 0) return 0;
+    kb_clear_cells<<<cells.blocks, kBatchBinBlock, 0, s>>>(bins, cells);
+    RCP_DISPATCH_ND(ndim, kb_bin_multi, <<<parts.blocks, kBatchBinBlock, 0, s>>>(bins, parts));
+    kb_scan_cells<<<(unsigned)nbins, kFoldBlock, 0, s>>>(bins);
+    kb_place<<<parts.blocks, kBatchBinBlock, 0, s>>>(bins, parts);
+    kb_cell_order<<<cells.blocks, kBatchBinBlock, 0, s>>>(bins, cells);
+    RCP_DISPATCH_ND(ndim, kb_permute, <<<parts.blocks, kBatchBinBlock, 0, s>>>(bins, parts));
     return check_launch();
 }
 

@@ -1,0 +1,466 @@
+// Surrogate network kernels (FP32): tiled GEMMs with fused bias / ReLU /
+// ReLU-derivative epilogues, split-K weight gradients, column sums, the
+// mean-squared-error head and Adam.  All sums run in a fixed order.
+#include "rcp_mlp.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+namespace rcp {
+
+namespace {
+
+constexpr int kGT = 64;      // output tile (rows and columns)
+constexpr int kGK = 16;      // k slice staged in LDS
+constexpr int kGThreads = 256;
+constexpr int kRedBlock = 256;
+constexpr int kLossBlock = 1024;
+
+enum Epilogue { kStore = 0, kBias = 1, kBiasRelu = 2, kReluMask = 3 };
+
+void check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// C = op(A) op(B) over k in [z * kchunk, min(K, (z + 1) * kchunk)) for
+// split z = blockIdx.z, written to C + z * split_stride.  op(A)(m, k) =
+// TA ? A[k * lda + m] : A[m * lda + k]; op(B)(k, n) = TB ? B[n * ldb + k]
+// : B[k * ldb + n].  Each thread owns a 4 x 4 block of the 64 x 64 tile.
+template <bool TA, bool TB, int EPI>
+__global__ __launch_bounds__(kGThreads) void k_gemm(int M, int N, int K, int kchunk,
+                                                    const float* __restrict__ A, int lda,
+                                                    const float* __restrict__ B, int ldb,
+                                                    float* __restrict__ C, int ldc,
+                                                    size_t split_stride,
+                                                    const float* __restrict__ bias,
+                                                    const float* __restrict__ mask, int ldm)
+{
+    __shared__ float As[kGK][kGT + 1];
+    __shared__ float Bs[kGK][kGT + 1];
+    const int tid = threadIdx.x;
+    const int m0 = blockIdx.y * kGT, n0 = blockIdx.x * kGT;
+    const int kb = blockIdx.z * kchunk;
+    const int ke = min(K, kb + kchunk);
+    const int tm = (tid / 16) * 4, tn = (tid % 16) * 4;
+    float acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
+    for (int k0 = kb; k0 < ke; k0 += kGK) {
+#pragma unroll
+        for (int r = 0; r < (kGT * kGK) / kGThreads; ++r) {
+            const int idx = tid + r * kGThreads;
+            const int mm = TA ? idx % kGT : idx / kGK;
+            const int kk = TA ? idx / kGT : idx % kGK;
+            const int gm = m0 + mm, gk = k0 + kk;
+            float v = 0.0f;
+            if (gm < M && gk < ke) v = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
+            As[kk][mm] = v;
+        }
+#pragma unroll
+        for (int r = 0; r < (kGT * kGK) / kGThreads; ++r) {
+            const int idx = tid + r * kGThreads;
+            const int nn = TB ? idx / kGK : idx % kGT;
+            const int kk = TB ? idx % kGK : idx / kGT;
+            const int gn = n0 + nn, gk = k0 + kk;
+            float v = 0.0f;
+            if (gn < N && gk < ke) v = TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
+            Bs[kk][nn] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kGK; ++kk) {
+            float a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[kk][tm + i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tn + j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+    float* Cz = C + (size_t)blockIdx.z * split_stride;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int gm = m0 + tm + i;
+        if (gm >= M) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int gn = n0 + tn + j;
+            if (gn >= N) continue;
+            float v = acc[i][j];
+            if (EPI == kBias || EPI == kBiasRelu) v += bias[gn];
+            if (EPI == kBiasRelu) v = v > 0.0f ? v : 0.0f;
+            if (EPI == kReluMask) v = mask[(size_t)gm * ldm + gn] > 0.0f ? v : 0.0f;
+            Cz[(size_t)gm * ldc + gn] = v;
+        }
+    }
+}
+
+// out[i] = sum over z = 0..S-1 of part[z * n + i]
+__global__ __launch_bounds__(kRedBlock) void k_sum_splits(const float* __restrict__ part, int S,
+                                                          size_t n, float* __restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * kRedBlock + threadIdx.x;
+    if (i >= n) return;
+    float s = part[i];
+    for (int z = 1; z < S; ++z) s += part[(size_t)z * n + i];
+    out[i] = s;
+}
+
+// Fixed-shape tree over a workgroup (thread 0 gets the total).
+template <int BLOCK>
+__device__ float block_sum(float v)
+{
+    __shared__ float sh[BLOCK];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = BLOCK / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    const float t = sh[0];
+    __syncthreads();
+    return t;
+}
+
+// db[n] = sum over m of dZ[m * N + n]; one workgroup per column
+__global__ __launch_bounds__(kRedBlock) void k_colsum(const float* __restrict__ dZ, int M, int N,
+                                                      float* __restrict__ db)
+{
+    const int n = blockIdx.x;
+    float s = 0.0f;
+    for (int m = threadIdx.x; m < M; m += kRedBlock) s += dZ[(size_t)m * N + n];
+    s = block_sum<kRedBlock>(s);
+    if (threadIdx.x == 0) db[n] = s;
+}
+
+// Mean squared error over count values: dY = 2 (Y - T) / count, *loss =
+// sum (Y - T)^2 / count (one workgroup)
+__global__ __launch_bounds__(kLossBlock) void k_mse(const float* __restrict__ Y,
+                                                    const float* __restrict__ T, int count,
+                                                    float* __restrict__ dY,
+                                                    float* __restrict__ loss)
+{
+    const float inv = 1.0f / (float)count;
+    float s = 0.0f;
+    for (int i = threadIdx.x; i < count; i += kLossBlock) {
+        const float e = Y[i] - T[i];
+        dY[i] = 2.0f * e * inv;
+        s += e * e;
+    }
+    s = block_sum<kLossBlock>(s);
+    if (threadIdx.x == 0) *loss = s * inv;
+}
+
+__global__ __launch_bounds__(kRedBlock) void k_adam(float* __restrict__ p,
+                                                    const float* __restrict__ g,
+                                                    float* __restrict__ m, float* __restrict__ v,
+                                                    size_t n, float lr, float inv_b1,
+                                                    float inv_b2)
+{
+    const size_t i = (size_t)blockIdx.x * kRedBlock + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = 0.9f * m[i] + 0.1f * gi;
+    const float vi = 0.999f * v[i] + 0.001f * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi * inv_b1) / (sqrtf(vi * inv_b2) + 1e-8f);
+}
+
+// out[r * w + c] = src[idx[r] * w + c]
+__global__ __launch_bounds__(kRedBlock) void k_gather_rows(const float* __restrict__ src,
+                                                           const uint32_t* __restrict__ idx,
+                                                           int rows, int w,
+                                                           float* __restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * kRedBlock + threadIdx.x;
+    if (i >= (size_t)rows * w) return;
+    const size_t r = i / w, c = i % w;
+    out[i] = src[(size_t)idx[r] * w + c];
+}
+
+unsigned blocks_for(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+template <bool TA, bool TB, int EPI>
+void gemm(hipStream_t s, int M, int N, int K, int splits, const float* A, int lda, const float* B,
+          int ldb, float* C, int ldc, const float* bias = nullptr, const float* mask = nullptr,
+          int ldm = 0)
+{
+    int kchunk = (K + splits - 1) / splits;
+    kchunk = (kchunk + kGK - 1) / kGK * kGK;
+    const dim3 grid(blocks_for(N, kGT), blocks_for(M, kGT), (unsigned)splits);
+    k_gemm<TA, TB, EPI><<<grid, kGThreads, 0, s>>>(M, N, K, kchunk, A, lda, B, ldb, C, ldc,
+                                                   (size_t)M * N, bias, mask, ldm);
+    check(hipGetLastError(), "gemm launch");
+}
+
+// splitmix64: the seeded generator of the initial weights and the shuffles
+uint64_t splitmix(uint64_t& s)
+{
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+double unit(uint64_t& s) { return (double)(splitmix(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+int splits_for(size_t B) { return (int)std::min<size_t>(32, std::max<size_t>(1, B / 512)); }
+
+}  // namespace
+
+Mlp::Mlp(int n_in, int n_hidden, int n_layers, int n_out, uint64_t seed)
+    : n_in_(n_in), n_hidden_(n_hidden), n_layers_(n_layers), n_out_(n_out)
+{
+    if (n_in < 1 || n_hidden < 1 || n_layers < 1 || n_out < 1 || n_in > 4096 ||
+        n_hidden > 4096 || n_out > 4096 || n_layers > 16)
+        throw std::invalid_argument("rcp: network shape out of range");
+    const int L = n_layers + 1;
+    for (int l = 0; l < L; ++l) {
+        const int fi = l == 0 ? n_in : n_hidden;
+        const int fo = l == L - 1 ? n_out : n_hidden;
+        fan_in_.push_back(fi);
+        fan_out_.push_back(fo);
+        w_off_.push_back(n_params_);
+        n_params_ += (size_t)fi * fo;
+        b_off_.push_back(n_params_);
+        n_params_ += (size_t)fo;
+    }
+    check(hipStreamCreate(&s_), "hipStreamCreate");
+    check(hipMalloc(&p_, n_params_ * sizeof(float)), "hipMalloc");
+    check(hipMalloc(&g_, n_params_ * sizeof(float)), "hipMalloc");
+    check(hipMalloc(&m_, n_params_ * sizeof(float)), "hipMalloc");
+    check(hipMalloc(&v_, n_params_ * sizeof(float)), "hipMalloc");
+    check(hipMalloc(&loss_, 4096 * sizeof(float)), "hipMalloc");
+    check(hipHostMalloc((void**)&h_loss_, 4096 * sizeof(float), hipHostMallocDefault),
+          "hipHostMalloc");
+    size_t maxw = 0;
+    for (int l = 0; l < L; ++l) maxw = std::max(maxw, (size_t)fan_in_[l] * fan_out_[l]);
+    check(hipMalloc(&part_, 32 * maxw * sizeof(float)), "hipMalloc");
+    // He-uniform weights, zero biases
+    std::vector<float> p(n_params_, 0.0f);
+    uint64_t st = seed;
+    for (int l = 0; l < L; ++l) {
+        const double a = std::sqrt(6.0 / fan_in_[l]);
+        for (size_t k = 0; k < (size_t)fan_in_[l] * fan_out_[l]; ++k)
+            p[w_off_[l] + k] = (float)((2.0 * unit(st) - 1.0) * a);
+    }
+    set_params(p.data());
+    act_.assign(L + 1, nullptr);
+}
+
+Mlp::~Mlp()
+{
+    if (s_) (void)hipStreamSynchronize(s_);
+    for (size_t l = 1; l < act_.size(); ++l) (void)hipFree(act_[l]);
+    for (float* q : {p_, g_, m_, v_, dA_, dB_, part_, loss_}) (void)hipFree(q);
+    if (h_loss_) (void)hipHostFree(h_loss_);
+    if (s_) (void)hipStreamDestroy(s_);
+}
+
+void Mlp::ensure_batch(size_t B)
+{
+    if (B <= cap_B_) return;
+    for (size_t l = 1; l < act_.size(); ++l) {
+        (void)hipFree(act_[l]);
+        act_[l] = nullptr;
+    }
+    (void)hipFree(dA_);
+    (void)hipFree(dB_);
+    dA_ = dB_ = nullptr;
+    const int L = n_layers_ + 1;
+    int widest = n_in_;
+    for (int l = 0; l < L; ++l) {
+        check(hipMalloc(&act_[l + 1], B * fan_out_[l] * sizeof(float)), "hipMalloc");
+        widest = std::max(widest, fan_out_[l]);
+    }
+    check(hipMalloc(&dA_, B * widest * sizeof(float)), "hipMalloc");
+    check(hipMalloc(&dB_, B * widest * sizeof(float)), "hipMalloc");
+    cap_B_ = B;
+}
+
+void Mlp::get_params(float* p) const
+{
+    check(hipMemcpy(p, p_, n_params_ * sizeof(float), hipMemcpyDeviceToHost), "download");
+}
+
+void Mlp::set_params(const float* p)
+{
+    check(hipMemcpy(p_, p, n_params_ * sizeof(float), hipMemcpyHostToDevice), "upload");
+    check(hipMemset(m_, 0, n_params_ * sizeof(float)), "memset");
+    check(hipMemset(v_, 0, n_params_ * sizeof(float)), "memset");
+    adam_t_ = 0;
+}
+
+void Mlp::forward_dev(const float* X, size_t B)
+{
+    const int L = n_layers_ + 1;
+    act_[0] = const_cast<float*>(X);
+    for (int l = 0; l < L; ++l) {
+        const int fi = fan_in_[l], fo = fan_out_[l];
+        if (l < L - 1)
+            gemm<false, true, kBiasRelu>(s_, (int)B, fo, fi, 1, act_[l], fi, p_ + w_off_[l], fi,
+                                         act_[l + 1], fo, p_ + b_off_[l]);
+        else
+            gemm<false, true, kBias>(s_, (int)B, fo, fi, 1, act_[l], fi, p_ + w_off_[l], fi,
+                                     act_[l + 1], fo, p_ + b_off_[l]);
+    }
+}
+
+// gradients of the last forward's loss against T (device, B x n_out); the
+// loss goes to loss_[0] ... set by the caller through slot
+void Mlp::backward_dev(const float* T, size_t B)
+{
+    const int L = n_layers_ + 1;
+    const int S = splits_for(B);
+    float* dz = dA_;
+    float* other = dB_;
+    for (int l = L - 1; l >= 0; --l) {
+        const int fi = fan_in_[l], fo = fan_out_[l];
+        (void)T;
+        // dW = dZ^T A_prev (fo x fi), split over the batch, summed in order
+        gemm<true, false, kStore>(s_, fo, fi, (int)B, S, dz, fo, act_[l], fi, part_, fi);
+        k_sum_splits<<<blocks_for((size_t)fo * fi, kRedBlock), kRedBlock, 0, s_>>>(
+            part_, S, (size_t)fo * fi, g_ + w_off_[l]);
+        k_colsum<<<(unsigned)fo, kRedBlock, 0, s_>>>(dz, (int)B, fo, g_ + b_off_[l]);
+        if (l > 0) {
+            // dA_prev = dZ W, through the ReLU of the previous layer
+            gemm<false, false, kReluMask>(s_, (int)B, fi, fo, 1, dz, fo, p_ + w_off_[l], fi,
+                                          other, fi, nullptr, act_[l], fi);
+            std::swap(dz, other);
+        }
+    }
+    check(hipGetLastError(), "backward launch");
+}
+
+void Mlp::adam_dev(float lr)
+{
+    ++adam_t_;
+    const float inv_b1 = (float)(1.0 / (1.0 - std::pow(0.9, (double)adam_t_)));
+    const float inv_b2 = (float)(1.0 / (1.0 - std::pow(0.999, (double)adam_t_)));
+    k_adam<<<blocks_for(n_params_, kRedBlock), kRedBlock, 0, s_>>>(p_, g_, m_, v_, n_params_, lr,
+                                                                   inv_b1, inv_b2);
+    check(hipGetLastError(), "adam launch");
+}
+
+void Mlp::predict(const float* X, size_t B, float* Y)
+{
+    if (B == 0) return;
+    ensure_batch(B);
+    float* dX = nullptr;
+    check(hipMalloc(&dX, B * n_in_ * sizeof(float)), "hipMalloc");
+    try {
+        check(hipMemcpyAsync(dX, X, B * n_in_ * sizeof(float), hipMemcpyHostToDevice, s_), "upload");
+        forward_dev(dX, B);
+        check(hipMemcpyAsync(Y, act_[n_layers_ + 1], B * n_out_ * sizeof(float),
+                             hipMemcpyDeviceToHost, s_), "download");
+        check(hipStreamSynchronize(s_), "sync");
+    } catch (...) {
+        (void)hipFree(dX);
+        throw;
+    }
+    (void)hipFree(dX);
+    act_[0] = nullptr;
+}
+
+void Mlp::gradients(const float* X, const float* T, size_t B, float* loss, float* grad)
+{
+    if (B == 0) throw std::invalid_argument("rcp: empty batch");
+    ensure_batch(B);
+    float *dX = nullptr, *dT = nullptr;
+    check(hipMalloc(&dX, B * n_in_ * sizeof(float)), "hipMalloc");
+    check(hipMalloc(&dT, B * n_out_ * sizeof(float)), "hipMalloc");
+    try {
+        check(hipMemcpyAsync(dX, X, B * n_in_ * sizeof(float), hipMemcpyHostToDevice, s_), "upload");
+        check(hipMemcpyAsync(dT, T, B * n_out_ * sizeof(float), hipMemcpyHostToDevice, s_), "upload");
+        forward_dev(dX, B);
+        k_mse<<<1, kLossBlock, 0, s_>>>(act_[n_layers_ + 1], dT, (int)(B * n_out_), dA_, loss_);
+        backward_dev(dT, B);
+        check(hipMemcpyAsync(h_loss_, loss_, sizeof(float), hipMemcpyDeviceToHost, s_), "download");
+        check(hipMemcpyAsync(grad, g_, n_params_ * sizeof(float), hipMemcpyDeviceToHost, s_),
+              "download");
+        check(hipStreamSynchronize(s_), "sync");
+    } catch (...) {
+        (void)hipFree(dX);
+        (void)hipFree(dT);
+        throw;
+    }
+    (void)hipFree(dX);
+    (void)hipFree(dT);
+    act_[0] = nullptr;
+    *loss = h_loss_[0];
+}
+
+void Mlp::train(const float* X, const float* T, size_t n, size_t batch, size_t epochs, float lr,
+                uint64_t seed, float* losses)
+{
+    if (n == 0 || batch == 0) throw std::invalid_argument("rcp: empty training set");
+    batch = std::min(batch, n);
+    const size_t per_epoch = (n + batch - 1) / batch;
+    if (per_epoch > 4096) throw std::invalid_argument("rcp: more than 4096 minibatches per epoch");
+    ensure_batch(batch);
+    float *dX = nullptr, *dT = nullptr, *bX = nullptr, *bT = nullptr;
+    uint32_t* didx = nullptr;
+    auto release = [&]() {
+        for (void* q : {(void*)dX, (void*)dT, (void*)bX, (void*)bT, (void*)didx}) (void)hipFree(q);
+    };
+    try {
+        check(hipMalloc(&dX, n * n_in_ * sizeof(float)), "hipMalloc");
+        check(hipMalloc(&dT, n * n_out_ * sizeof(float)), "hipMalloc");
+        check(hipMalloc(&bX, batch * n_in_ * sizeof(float)), "hipMalloc");
+        check(hipMalloc(&bT, batch * n_out_ * sizeof(float)), "hipMalloc");
+        check(hipMalloc(&didx, n * sizeof(uint32_t)), "hipMalloc");
+        check(hipMemcpyAsync(dX, X, n * n_in_ * sizeof(float), hipMemcpyHostToDevice, s_), "upload");
+        check(hipMemcpyAsync(dT, T, n * n_out_ * sizeof(float), hipMemcpyHostToDevice, s_), "upload");
+        std::vector<uint32_t> idx(n);
+        uint64_t st = seed;
+        for (size_t e = 0; e < epochs; ++e) {
+            for (size_t i = 0; i < n; ++i) idx[i] = (uint32_t)i;
+            for (size_t i = n - 1; i > 0; --i) {
+                const size_t j = (size_t)(splitmix(st) % (i + 1));
+                std::swap(idx[i], idx[j]);
+            }
+            check(hipMemcpyAsync(didx, idx.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                 s_), "upload");
+            for (size_t b = 0; b < per_epoch; ++b) {
+                const size_t r0 = b * batch;
+                const size_t rows = std::min(batch, n - r0);
+                k_gather_rows<<<blocks_for(rows * n_in_, kRedBlock), kRedBlock, 0, s_>>>(
+                    dX, didx + r0, (int)rows, n_in_, bX);
+                k_gather_rows<<<blocks_for(rows * n_out_, kRedBlock), kRedBlock, 0, s_>>>(
+                    dT, didx + r0, (int)rows, n_out_, bT);
+                forward_dev(bX, rows);
+                k_mse<<<1, kLossBlock, 0, s_>>>(act_[n_layers_ + 1], bT, (int)(rows * n_out_), dA_,
+                                                loss_ + b);
+                backward_dev(bT, rows);
+                adam_dev(lr);
+            }
+            check(hipMemcpyAsync(h_loss_, loss_, per_epoch * sizeof(float), hipMemcpyDeviceToHost,
+                                 s_), "download");
+            check(hipStreamSynchronize(s_), "sync");
+            double sum = 0.0;
+            for (size_t b = 0; b < per_epoch; ++b) sum += h_loss_[b];
+            if (losses) losses[e] = (float)(sum / (double)per_epoch);
+        }
+    } catch (...) {
+        release();
+        act_[0] = nullptr;
+        throw;
+    }
+    release();
+    act_[0] = nullptr;
+}
+
+}  // namespace rcp

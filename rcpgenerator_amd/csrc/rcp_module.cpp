@@ -128,9 +128,11 @@ void parse_run(const py::dict& input, const py::dict& config, const py::dict& op
     c.neighbor_max = neighbor_max;
     c.seed = seed;
     c.fix_height = fix_height;
-    size_t max_steps = 0, reorder_interval = 2000;
+    size_t max_steps = 0, reorder_interval = 2000, rung1_deadline = 0;
     bool fix_diameter = false;
     take(options, "max_steps", max_steps);
+    take(options, "rung1_deadline", rung1_deadline);
+    c.rung1_deadline = rung1_deadline;
     take(options, "fix_diameter", fix_diameter);
     take(options, "reorder_interval", reorder_interval);
     c.max_steps = max_steps;
@@ -155,30 +157,49 @@ void parse_run(const py::dict& input, const py::dict& config, const py::dict& op
     c.neighbor_mode = neighbor_mode;
 }
 
-py::dict result_dict(rcp_engine* e)
+// Result arrays of a finished engine (no Python objects: also filled on the
+// worker threads of a batch).
+struct ResultData {
+    int rc = RCP_OK;
+    std::string err;
+    rcp_result_info info{};
+    std::vector<double> x, D, box, ph, F, U, ov, mu, al;
+    std::vector<uint64_t> origin;
+    std::vector<int8_t> walls, fl;
+};
+
+int collect_result(rcp_engine* e, ResultData& r)
 {
-    rcp_result_info info;
-    int rc = rcp_engine_info(e, &info);
-    if (rc) raise_rc(rc);
-    const size_t N = info.N, nd = info.ndim, s = info.steps;
-    std::vector<double> x(N * nd), D(N), box(nd), ph(s), F(s), U(s), ov(s), mu(s), al(s);
-    std::vector<uint64_t> origin(N);
-    std::vector<int8_t> walls(nd), fl(s);
-    rc = rcp_engine_result(e, x.data(), D.data(), origin.data(), box.data(), walls.data(),
-                           ph.data(), F.data(), U.data(), ov.data(), mu.data(), al.data(), fl.data());
-    if (rc) raise_rc(rc);
+    int rc = rcp_engine_info(e, &r.info);
+    if (rc) return rc;
+    const size_t N = r.info.N, nd = r.info.ndim, s = r.info.steps;
+    r.x.resize(N * nd);
+    r.D.resize(N);
+    r.box.resize(nd);
+    for (auto* h : {&r.ph, &r.F, &r.U, &r.ov, &r.mu, &r.al}) h->resize(s);
+    r.origin.resize(N);
+    r.walls.resize(nd);
+    r.fl.resize(s);
+    return rcp_engine_result(e, r.x.data(), r.D.data(), r.origin.data(), r.box.data(),
+                             r.walls.data(), r.ph.data(), r.F.data(), r.U.data(), r.ov.data(),
+                             r.mu.data(), r.al.data(), r.fl.data());
+}
+
+py::dict to_dict(const ResultData& r)
+{
+    const size_t N = r.info.N, nd = r.info.ndim, s = r.info.steps;
     py::dict out;
-    out["positions"] = list_nested(x.data(), N, nd);
-    out["diameters"] = list_f64(D.data(), N);
-    out["box"] = list_f64(box.data(), nd);
-    out["walls"] = list_int(walls.data(), nd);
-    out["phi_history"] = list_f64(ph.data(), s);
-    out["force_history"] = list_f64(F.data(), s);
-    out["energy_history"] = list_f64(U.data(), s);
-    out["max_overlap_history"] = list_f64(ov.data(), s);
-    out["mu_history"] = list_f64(mu.data(), s);
-    out["alpha_history"] = list_f64(al.data(), s);
-    out["mu_flag_history"] = list_int(fl.data(), s);
+    out["positions"] = list_nested(r.x.data(), N, nd);
+    out["diameters"] = list_f64(r.D.data(), N);
+    out["box"] = list_f64(r.box.data(), nd);
+    out["walls"] = list_int(r.walls.data(), nd);
+    out["phi_history"] = list_f64(r.ph.data(), s);
+    out["force_history"] = list_f64(r.F.data(), s);
+    out["energy_history"] = list_f64(r.U.data(), s);
+    out["max_overlap_history"] = list_f64(r.ov.data(), s);
+    out["mu_history"] = list_f64(r.mu.data(), s);
+    out["alpha_history"] = list_f64(r.al.data(), s);
+    out["mu_flag_history"] = list_int(r.fl.data(), s);
     // research instruments the reference leaves empty by default (MOD:169-177)
     out["pp_F_sq_sum"] = py::list();
     out["pp_F_sign_flips"] = py::list();
@@ -189,11 +210,27 @@ py::dict result_dict(rcp_engine* e)
     out["spike_log"] = py::list();
     out["reorder_log"] = py::list();
     out["steps"] = s;
-    out["phi"] = info.phi;
-    out["max_min_dist"] = info.max_min_dist;
-    out["force_magnitude"] = info.force_magnitude;
-    out["particle_origin"] = list_int(origin.data(), N);
+    out["phi"] = r.info.phi;
+    out["max_min_dist"] = r.info.max_min_dist;
+    out["force_magnitude"] = r.info.force_magnitude;
+    out["particle_origin"] = list_int(r.origin.data(), N);
     return out;
+}
+
+py::dict result_dict(rcp_engine* e)
+{
+    ResultData r;
+    int rc = collect_result(e, r);
+    if (rc) raise_rc(rc);
+    return to_dict(r);
+}
+
+void batch_sink(void* user, size_t index, int rc, rcp_engine* e)
+{
+    ResultData& r = (*static_cast<std::vector<ResultData>*>(user))[index];
+    r.rc = rc;
+    if (rc == RCP_OK) r.rc = collect_result(e, r);
+    if (r.rc != RCP_OK) r.err = rcp_last_error();
 }
 
 py::dict trace_dict(rcp_engine* e)
@@ -479,6 +516,56 @@ PYBIND11_MODULE(_rcpgenerator, m)
         py::arg("input"), py::arg("config"), py::arg("progress_interval") = 0,
         py::arg("capture_positions") = false, py::arg("trajectory_interval") = 0,
         py::arg("callback") = py::none(), py::arg("options") = py::dict());
+
+    m.def(
+        "run_packing_batch",
+        [](const py::list& inputs, py::object configs, py::object options, int threads,
+           bool pool) {
+            // one (input, config, options) per packing; a single dict for
+            // configs or options applies to every packing
+            const size_t n = inputs.size();
+            auto pick = [&](py::object o, size_t i) -> py::dict {
+                if (o.is_none()) return py::dict();
+                if (py::isinstance<py::dict>(o)) return o.cast<py::dict>();
+                return o.cast<py::list>()[i].cast<py::dict>();
+            };
+            std::vector<RunInputs> runs(n);
+            std::vector<rcp_run_config> cfgs(n);
+            for (size_t i = 0; i < n; ++i) {
+                parse_run(inputs[i].cast<py::dict>(), pick(configs, i), pick(options, i), runs[i]);
+                cfgs[i] = runs[i].cfg;
+            }
+            std::vector<ResultData> res(n);
+            {
+                py::gil_scoped_release nogil;
+                rcp_run_batch(cfgs.data(), n, threads, pool ? RCP_BATCH_POOL : 0, batch_sink,
+                              &res);
+            }
+            py::list out;
+            for (size_t i = 0; i < n; ++i) {
+                if (res[i].rc != RCP_OK)
+                    throw std::runtime_error("packing " + std::to_string(i) + ": " + res[i].err);
+                out.append(to_dict(res[i]));
+            }
+            return out;
+        },
+        py::arg("inputs"), py::arg("configs") = py::none(), py::arg("options") = py::none(),
+        py::arg("threads") = 0, py::arg("pool") = false,
+        "Runs independent packings to completion on the GPU and returns one run_packing "
+        "result dict per input.  Packings the batched engine supports step together in "
+        "shared launches (size-class cells); the others, or all with pool=True, run on "
+        "`threads` host workers with one engine each.");
+
+    m.def("last_batch_stats", []() {
+        uint64_t rounds = 0, steps = 0;
+        double ms = 0.0;
+        rcp_batch_last_stats(&rounds, &steps, &ms);
+        py::dict d;
+        d["rounds"] = rounds;
+        d["packing_steps"] = steps;
+        d["device_ms"] = ms;
+        return d;
+    }, "Counters of the last run_packing_batch on this thread (batched engine only).");
 
     m.def("device_info", []() {
         char buf[256];

@@ -1,0 +1,100 @@
+"""The surrogate network's HIP kernels against the CPU oracle (oracle/mlp.py):
+forward outputs, back-propagated gradients and one Adam step, FP32 on the
+GPU against float64 on the CPU (tolerances below), plus reproducible and
+converging training."""
+import numpy as np
+import pytest
+
+from oracle import mlp as M
+
+pytestmark = pytest.mark.gpu
+
+# FP32 against float64: a 128-wide layer accumulates 128 products per
+# output, so relative errors stay near 1e-6; 2e-4 leaves room for the
+# gradient sums over a batch
+RTOL = 2e-4
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from rcpgenerator_amd import capi as c
+    return c
+
+
+def _problem(seed, n_in=11, B=300):
+    r = np.random.default_rng(seed)
+    X = r.normal(size=(B, n_in)).astype(np.float32)
+    T = r.normal(size=(B, 2)).astype(np.float32)
+    return X, T
+
+
+@pytest.mark.parametrize("shape,B", [((11, 128, 3, 2), 300), ((5, 70, 2, 3), 1), ((16, 128, 1, 1), 2050)])
+def test_forward_and_gradients_match_oracle(capi, shape, B):
+    net = capi.Mlp(*shape, seed=7)
+    p = net.get_params().astype(np.float64)
+    r = np.random.default_rng(B)
+    X = r.normal(size=(B, shape[0])).astype(np.float32)
+    T = r.normal(size=(B, shape[3])).astype(np.float32)
+    Y = net.predict(X)
+    Yo, _ = M.forward(p, X.astype(np.float64), *shape)
+    assert np.allclose(Y, Yo, rtol=RTOL, atol=RTOL * np.abs(Yo).max())
+    loss, g = net.gradients(X, T)
+    lo, go = M.gradients(p, X.astype(np.float64), T.astype(np.float64), *shape)
+    assert loss == pytest.approx(lo, rel=RTOL)
+    scale = np.abs(go).max()
+    assert np.allclose(g, go, rtol=RTOL, atol=RTOL * scale)
+
+
+def test_initial_weights_are_seeded(capi):
+    a = capi.Mlp(11, 128, 3, 2, seed=3).get_params()
+    b = capi.Mlp(11, 128, 3, 2, seed=3).get_params()
+    c = capi.Mlp(11, 128, 3, 2, seed=4).get_params()
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    n = 11 * 128 + 128 + 2 * (128 * 128 + 128) + 128 * 2 + 2
+    assert a.size == n
+    # He-uniform bound of the first layer, zero biases
+    assert np.abs(a[:11 * 128]).max() <= np.sqrt(6.0 / 11) and not a[11 * 128:11 * 128 + 128].any()
+
+
+def test_one_training_step_is_adam_on_the_gradient(capi):
+    """A single full-batch epoch of train() = one Adam step from the
+    gradient at the starting parameters."""
+    X, T = _problem(1, B=200)
+    net = capi.Mlp(11, 128, 3, 2, seed=9)
+    p0 = net.get_params()
+    _, g = net.gradients(X, T)
+    net.train(X, T, batch=200, epochs=1, lr=1e-3, seed=0)
+    p1 = net.get_params()
+    z = np.zeros_like(p0, dtype=np.float64)
+    pe, _, _ = M.adam_step(p0.astype(np.float64), g.astype(np.float64), z, z, 1, lr=1e-3)
+    assert np.allclose(p1, pe, rtol=1e-5, atol=2e-6)
+
+
+def test_training_is_reproducible_and_converges(capi):
+    r = np.random.default_rng(11)
+    X = r.uniform(-1, 1, size=(4000, 6)).astype(np.float32)
+    T = np.stack([np.sin(2 * X[:, 0]) + X[:, 1] * X[:, 2], X[:, 3] ** 2 - X[:, 4]],
+                 axis=1).astype(np.float32)
+    runs = []
+    for _ in range(2):
+        net = capi.Mlp(6, 128, 3, 2, seed=1)
+        losses = net.train(X, T, batch=256, epochs=60, lr=2e-3, seed=5)
+        runs.append((losses, net.get_params()))
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+    losses = runs[0][0]
+    assert losses[-1] < 0.05 * losses[0]
+    # held-out error of the trained network
+    Xt = r.uniform(-1, 1, size=(500, 6)).astype(np.float32)
+    Tt = np.stack([np.sin(2 * Xt[:, 0]) + Xt[:, 1] * Xt[:, 2], Xt[:, 3] ** 2 - Xt[:, 4]], axis=1)
+    net = capi.Mlp(6, 128, 3, 2, seed=1)
+    net.set_params(runs[0][1])
+    err = np.mean((net.predict(Xt) - Tt) ** 2)
+    assert err < 0.05 * np.var(Tt)
+
+
+def test_bad_shapes_raise(capi):
+    with pytest.raises(capi.RcpError):
+        capi.Mlp(0, 128, 3, 2)
+    net = capi.Mlp(3, 8, 1, 1)
+    with pytest.raises(ValueError):
+        net.set_params(np.zeros(5))

@@ -474,3 +474,127 @@ def test_observed_trajectory_matches_stepwise(gpu):
     e.run(1000)
     k = steps.index(1000)
     assert same(np.array(out["trace"]["positions"][k]), e.result()["positions"])
+
+
+# ------------------------------------------------------------- batched runs
+
+def _batch_cases():
+    import rcpgenerator_amd as rcp
+    specs = [
+        (300, 2, [0, 0], {"type": "mono", "d": 1.0}, 0.1),
+        (900, 3, [0, 0, 1], {"type": "bidisperse", "d1": 1.0, "d2": 1.4, "p": 0.5}, 0.08),
+        (500, 2, [1, 0], {"type": "lognormal", "mu": 0.0, "sigma": 0.25}, 0.12),
+        (400, 4, [0, 0, 0, 0], {"type": "mono", "d": 1.0}, 0.05),
+        (1200, 3, [0, 0, 0], {"type": "powerlaw", "d_min": 0.2, "d_max": 1.0, "exponent": -2.5},
+         0.1),
+        (250, 2, [-1, -1], {"type": "flat", "d_min": 0.5, "d_max": 1.5}, 0.1),
+    ]
+    out = []
+    for k, (N, nd, walls, dist, phi) in enumerate(specs):
+        init = rcp.initialize_particles({"phi": phi, "N": N, "Ndim": nd, "box": [1.0] * nd,
+                                         "walls": walls, "seed": k, "dist": dist})
+        out.append(({"positions": init["positions"], "diameters": init["diameters"]},
+                    {"box": init["box"], "walls": init["walls"]}))
+    return out
+
+
+def _assert_same_run(r, s):
+    assert r["steps"] == s["steps"]
+    assert same(r["positions"], s["positions"]) and same(r["diameters"], s["diameters"])
+    assert r["particle_origin"] == s["particle_origin"]
+    assert r["phi"] == s["phi"] and r["max_min_dist"] == s["max_min_dist"]
+    for key in HIST:
+        assert same(r[key], s[key]), key
+    assert r["mu_flag_history"] == s["mu_flag_history"]
+
+
+def test_batch_pool_matches_single_runs(gpu):
+    """pool=True runs each packing on its own engine, as run_packing does:
+    every result is identical to the single run's."""
+    import rcpgenerator_amd as rcp
+    cases = _batch_cases()
+    opts = {"max_steps": 2500}
+    batch = rcp.run_packing_batch([c[0] for c in cases], [c[1] for c in cases], opts, threads=3,
+                                  pool=True)
+    assert len(batch) == len(cases)
+    for (inp, cfg), r in zip(cases, batch):
+        _assert_same_run(r, rcp.run_packing_observed(inp, cfg, options=opts)["result"])
+
+
+def _wide_cases():
+    """Power-law packings with three and four size classes (ratio 5 and 10):
+    their largest classes take the one-wavefront-per-particle kernel."""
+    import rcpgenerator_amd as rcp
+    out = []
+    for k, (N, nd, walls, dmin) in enumerate([(1500, 3, [0, 0, 1], 0.1), (1200, 2, [0, 0], 0.2),
+                                              (900, 4, [0, 0, 0, 1], 0.1)]):
+        init = rcp.initialize_particles({"phi": 0.08, "N": N, "Ndim": nd, "box": [1.0] * nd,
+                                         "walls": walls, "seed": 40 + k,
+                                         "dist": {"type": "powerlaw", "d_min": dmin,
+                                                  "d_max": 1.0, "exponent": -2.5}})
+        out.append(({"positions": init["positions"], "diameters": init["diameters"]},
+                    {"box": init["box"], "walls": init["walls"]}))
+    return out
+
+
+def test_batched_engine_matches_single_size_class_runs(gpu):
+    """The batched engine (shared launches for all packings of a dimension)
+    gives every packing the arithmetic, partial rows and fold of its single
+    run in size-class mode: identical results, through the reference
+    reorder at step 2000 and with the one-wave kernel for wide classes."""
+    import rcpgenerator_amd as rcp
+    cases = _batch_cases() + _wide_cases()
+    opts = {"max_steps": 2300}
+    batch = rcp.run_packing_batch([c[0] for c in cases], [c[1] for c in cases], opts)
+    st = rcp.last_batch_stats()
+    assert st["rounds"] == 2300 and st["packing_steps"] == 2300 * len(cases)
+    for (inp, cfg), r in zip(cases, batch):
+        s = rcp.run_packing_observed(inp, cfg, options=dict(opts, neighbor_mode=4))["result"]
+        _assert_same_run(r, s)
+
+
+def test_batched_engine_full_runs(gpu):
+    """Packings that stop at different steps (convergence gate) leave the
+    shared launches one by one; each result equals its single run's."""
+    import rcpgenerator_amd as rcp
+    cases = _batch_cases()[:3]
+    batch = rcp.run_packing_batch([c[0] for c in cases], [c[1] for c in cases])
+    steps = [r["steps"] for r in batch]
+    assert len(set(steps)) > 1
+    assert rcp.last_batch_stats()["rounds"] == max(steps)
+    for (inp, cfg), r in zip(cases, batch):
+        s = rcp.run_packing_observed(inp, cfg, options={"neighbor_mode": 4})["result"]
+        _assert_same_run(r, s)
+        assert r["max_min_dist"] < 5e-4
+
+
+def test_batch_mixed_routes(gpu):
+    """Packings outside the batched engine (exact reductions, forced list
+    mode) run on the engine pool in the same call; a packing whose set-up
+    fails is reported by its index."""
+    import rcpgenerator_amd as rcp
+    cases = _batch_cases()[:4]
+    opts = [{"max_steps": 600}, {"max_steps": 600, "exact_reductions": True},
+            {"max_steps": 600, "neighbor_mode": 1}, {"max_steps": 600}]
+    batch = rcp.run_packing_batch([c[0] for c in cases], [c[1] for c in cases], opts, threads=2)
+    for (inp, cfg), o, r in zip(cases, opts, batch):
+        oo = dict(o) if "exact_reductions" in o or "neighbor_mode" in o else dict(o, neighbor_mode=4)
+        _assert_same_run(r, rcp.run_packing_observed(inp, cfg, options=oo)["result"])
+    bad = [dict(cases[0][1]), dict(cases[1][1])]
+    bad[1]["box"] = [1.0, 1.0]   # 3-D packing, 2 box lengths
+    with pytest.raises(RuntimeError, match="packing 1: Error: --box length mismatch"):
+        rcp.run_packing_batch([cases[0][0], cases[1][0]], bad, {"max_steps": 50})
+
+
+def test_batch_exact_mode_vs_oracle(gpu, oracle):
+    import rcpgenerator_amd as rcp
+    cases = _batch_cases()[:3]
+    batch = rcp.run_packing_batch([c[0] for c in cases], [c[1] for c in cases],
+                                  {"max_steps": 1500, "exact_reductions": True}, threads=2)
+    for (inp, cfg), r in zip(cases, batch):
+        o = oracle.run_packing(inp["positions"], inp["diameters"], box=cfg["box"],
+                               walls=cfg["walls"], max_steps=1500)
+        assert r["steps"] == o["steps"]
+        assert same(r["positions"], o["positions"])
+        for key in HIST:
+            assert same(r[key], o[key]), key

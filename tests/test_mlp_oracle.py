@@ -1,0 +1,70 @@
+"""The surrogate network's CPU oracle (oracle/mlp.py) pinned without a GPU:
+back-propagated gradients against central finite differences, and Adam's
+first step against its closed form."""
+import numpy as np
+import pytest
+
+from oracle import mlp as M
+
+SHAPE = (5, 7, 3, 2)   # n_in, n_hidden, n_layers, n_out
+
+
+def _data(seed=0, B=9):
+    r = np.random.default_rng(seed)
+    n = sum(fi * fo + fo for fi, fo in M.shapes(*SHAPE))
+    p = r.normal(0.0, 0.6, n)
+    X = r.normal(0.0, 1.0, (B, SHAPE[0]))
+    T = r.normal(0.0, 1.0, (B, SHAPE[3]))
+    return p, X, T
+
+
+def test_parameter_count_and_layout():
+    p, X, T = _data()
+    layers = M.unpack(p, *SHAPE)
+    assert [W.shape for W, _ in layers] == [(7, 5), (7, 7), (7, 7), (2, 7)]
+    # W of layer 0 is the first 35 entries, row-major over the inputs
+    assert np.array_equal(layers[0][0][1], p[5:10])
+    assert np.array_equal(layers[0][1], p[35:42])
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_gradients_match_finite_differences(seed):
+    p, X, T = _data(seed)
+    loss, g = M.gradients(p, X, T, *SHAPE)
+    assert loss == pytest.approx(M.loss(p, X, T, *SHAPE), rel=1e-14)
+    h = 1e-6
+    fd = np.zeros_like(p)
+    for k in range(p.size):
+        e = np.zeros_like(p)
+        e[k] = h
+        fd[k] = (M.loss(p + e, X, T, *SHAPE) - M.loss(p - e, X, T, *SHAPE)) / (2 * h)
+    # ReLU kinks: no pre-activation lies within h of zero for these draws
+    assert np.allclose(g, fd, rtol=1e-6, atol=1e-8)
+
+
+def test_gradient_of_linear_head_closed_form():
+    """One linear layer (no hidden ReLU): dL/dW = 2/(B n_out) E^T X."""
+    r = np.random.default_rng(5)
+    X = r.normal(size=(6, 3))
+    T = r.normal(size=(6, 2))
+    W = r.normal(size=(2, 3))
+    b = r.normal(size=2)
+    E = X @ W.T + b - T
+    # a 0-hidden-layer net is n_layers=0 in shapes(): emulate with one layer
+    fi_fo = M.shapes(3, 4, 0, 2)
+    assert fi_fo == [(3, 2)]
+    p = np.concatenate([W.ravel(), b])
+    loss, g = M.gradients(p, X, T, 3, 4, 0, 2)
+    assert loss == pytest.approx(np.mean(E ** 2))
+    assert np.allclose(g[:6].reshape(2, 3), 2.0 / E.size * E.T @ X)
+    assert np.allclose(g[6:], 2.0 / E.size * E.sum(axis=0))
+
+
+def test_adam_first_step_closed_form():
+    r = np.random.default_rng(3)
+    p = r.normal(size=20)
+    g = r.normal(size=20)
+    z = np.zeros(20)
+    p1, m1, v1 = M.adam_step(p, g, z, z, 1, lr=1e-2)
+    assert np.allclose(p1, p - 1e-2 * g / (np.abs(g) + 1e-8), rtol=1e-12, atol=1e-15)
+    assert np.allclose(m1, 0.1 * g) and np.allclose(v1, 0.001 * g * g)
