@@ -102,7 +102,8 @@ private:
     double* D0_[2] = {};
     uint32_t* tag_[2] = {};
     uint32_t *inv_tag_ = nullptr, *tag_tmp_ = nullptr, *cell_of_ = nullptr, *slot_ = nullptr,
-             *perm_ = nullptr, *cell_count_ = nullptr, *cell_start_ = nullptr;
+             *perm_ = nullptr, *cell_count_ = nullptr, *cell_start_ = nullptr,
+             *big_ = nullptr, *big_count_ = nullptr;
     unsigned long long* origin_[2] = {};
     double* partials_ = nullptr;
     double* results_ = nullptr;
@@ -172,6 +173,8 @@ void Group::allocate()
                  ocof = take(NT * sizeof(uint32_t)), oslot = take(NT * sizeof(uint32_t)),
                  operm = take(NT * sizeof(uint32_t));
     const size_t occ = take(CT * sizeof(uint32_t)), ocs = take(CT * sizeof(uint32_t));
+    const size_t obig = take(2 * (NT / 16 + m_.size() + 1) * sizeof(uint32_t)),
+                 obc = take(sizeof(uint32_t));
     const size_t opart = take(rows * S_NUM * sizeof(double));
     const size_t ores = take(m_.size() * S_NUM * sizeof(double));
     const size_t otx = take(maxN * d * sizeof(double)), otx2 = take(maxN * sizeof(double));
@@ -198,6 +201,8 @@ void Group::allocate()
     perm_ = reinterpret_cast<uint32_t*>(b + operm);
     cell_count_ = reinterpret_cast<uint32_t*>(b + occ);
     cell_start_ = reinterpret_cast<uint32_t*>(b + ocs);
+    big_ = reinterpret_cast<uint32_t*>(b + obig);
+    big_count_ = reinterpret_cast<uint32_t*>(b + obc);
     partials_ = reinterpret_cast<double*>(b + opart);
     results_ = reinterpret_cast<double*>(b + ores);
     tmp_ = reinterpret_cast<double*>(b + otx);
@@ -342,6 +347,8 @@ void Group::stage_and_launch(bool do_adam)
             B.b.cell_count = cell_count_ + M.coff;
             B.b.cell_start = cell_start_ + M.coff;
             B.b.perm = perm_ + p;
+            B.b.big = big_;
+            B.b.big_count = big_count_;
             B.mg = M.mg;
             B.anchor_o = l.anchor ? anchor_ + pd : nullptr;
             part_pre[nbins] = pb;
@@ -413,7 +420,7 @@ void Group::stage_and_launch(bool do_adam)
     hip_check(hipEventRecord(ev0_, s_), "event");
     if (nbins) {
         if (launch_batch_rebin(nd_, dbins, nbins, table(3, nbins, pb, false),
-                               table(4, nbins, cb, false), s_))
+                               table(4, nbins, cb, false), big_, big_count_, s_))
             throw HipFailure("batched rebin launch");
     }
     if (launch_batch_step(nd_, dpacks, n, table(0, n, sb, false), table(1, nwide, wb, true),

@@ -160,6 +160,8 @@ DeviceState::DeviceState(int N_, int nd_, bool with_fixed, bool exact_, hipStrea
         cell_of = alloc<uint32_t>(N);
         slot = alloc<uint32_t>(N);
         perm = alloc<uint32_t>(N);
+        big = alloc<uint32_t>(2 * ((size_t)N / 16 + 1));
+        big_count = alloc<uint32_t>(1);
         row_count = alloc<uint32_t>((size_t)N + 1);
         row_start = alloc<uint32_t>((size_t)N + 1);
         sort.keys = alloc<unsigned long long>(N);
@@ -246,6 +248,8 @@ void DeviceState::rebin(const Geometry& g, double reach)
     b.cell_count = cell_count;
     b.cell_start = cell_start;
     b.perm = perm;
+    b.big = big;
+    b.big_count = big_count;
     if (launch_rebin(b, scan_tmp, s)) throw HipFailure("rebin launch");
     std::swap(x, x_alt);
     std::swap(m, m_alt);
@@ -292,6 +296,8 @@ void DeviceState::rebin_multi(const Geometry& g, const MultiGrid& mg, double kap
     b.cell_count = cell_count;
     b.cell_start = cell_start;
     b.perm = perm;
+    b.big = big;
+    b.big_count = big_count;
     if (launch_rebin_multi(b, mg, scan_tmp, s)) throw HipFailure("size-class rebin launch");
     std::swap(x, x_alt);
     std::swap(m, m_alt);
@@ -432,7 +438,11 @@ Engine::Engine(const rcp_run_config& c, Hosted)
     if (c.fixed)
         for (std::size_t i = 0; i < c.N; ++i)
             if (c.fixed[i]) throw ArgError("rcp: runs with frozen diameters are not batched");
-    setup(c, true);
+    if (c.neighbor_mode != 0 && c.neighbor_mode != 4)
+        throw ArgError("rcp: batched runs use size-class cells (neighbor_mode 0 or 4)");
+    rcp_run_config cc = c;
+    cc.neighbor_mode = 4;   // decisions of the size-class mode (binning_due)
+    setup(cc, true);
 }
 
 void Engine::setup(const rcp_run_config& c, bool hosted)
@@ -792,7 +802,21 @@ bool Engine::binning_due(double growth_step, double& skin) const
     // a skin pays off once a binning lives two steps or more
     const double per_step = 2.0 * std::sqrt(maxstep2_) + growth_step + 2.0 * fh_step_;
     const double shell = shell0_ * kappa_D_;
-    skin = per_step * 2.0 < shell ? shell : 0.0;
+    if (!multi_supported()) {
+        skin = per_step * 2.0 < shell ? shell : 0.0;
+        return true;
+    }
+    // Size-class cells: the candidates of a step grow as (1 + skin/w)^d for
+    // cells of width w (about dmax), and a binning lives skin / per_step
+    // steps.  With a binning costing rho candidate scans of a step, the
+    // per-step cost (1 + s)^d + rho per_step / (s w) is least near
+    // s = sqrt(rho per_step / (d w)) (small s); rho = 0.6 / d (the binning
+    // passes weigh less against the wider stencils of higher dimensions).
+    const double w = dmax_now();
+    const double d = static_cast<double>(nd_);
+    const double rho = 0.6 / d;
+    skin = (w > 0.0) ? std::min(shell, std::sqrt(rho * per_step * w / d)) : 0.0;
+    if (!(per_step * 2.0 < skin)) skin = 0.0;
     return true;
 }
 

@@ -95,6 +95,7 @@ public:
     // binning / rows
     uint32_t *cell_of = nullptr, *slot = nullptr, *perm = nullptr, *cell_count = nullptr,
              *cell_start = nullptr, *row_count = nullptr, *row_start = nullptr, *nbr = nullptr;
+    uint32_t *big = nullptr, *big_count = nullptr;   // crowded cells of a binning
     size_t cell_cap = 0, nbr_cap = 0;
     // exact reductions
     double *contrib = nullptr, *mag = nullptr, *ref_terms = nullptr;

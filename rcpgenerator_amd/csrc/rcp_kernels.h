@@ -162,6 +162,10 @@ struct BinArgs {
     uint32_t* cell_count = nullptr;    // total + 1
     uint32_t* cell_start = nullptr;    // total + 1
     uint32_t* perm = nullptr;          // N: old position of new position q
+    // optional list of cells with more than 16 members (2 u32 per entry,
+    // room for N / 16 + 1 entries) and its counter; null: sorted in place
+    uint32_t* big = nullptr;
+    uint32_t* big_count = nullptr;
 };
 int launch_rebin(BinArgs& a, void* scan_tmp, hipStream_t s);
 // Re-binning by size class (a.grid.total = mg.base[mg.C]).
@@ -213,9 +217,12 @@ int launch_batch_step(int ndim, const BatchPack* packs, int npacks, const BatchT
                       const BatchTable& wide, double* results, hipStream_t s,
                       hipEvent_t mark = nullptr);
 // Re-binning of the packings in bins[0..nbins): `parts` tiles their
-// particles, `cells` their cell tables (total + 1 entries each).
+// particles, `cells` their cell tables (total + 1 entries each); big /
+// big_count: the list of crowded cells (2 u32 per entry, room for the
+// packings' N / 16 + 1 entries) and its counter.
 int launch_batch_rebin(int ndim, const BatchBin* bins, int nbins, const BatchTable& parts,
-                       const BatchTable& cells, hipStream_t s);
+                       const BatchTable& cells, uint32_t* big, uint32_t* big_count,
+                       hipStream_t s);
 
 // ---------------------------------------------------------------- lists
 struct ListArgs {

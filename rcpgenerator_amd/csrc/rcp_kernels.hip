@@ -954,6 +954,32 @@ __global__ __launch_bounds__(BLOCK) void k_step_tile(StepArgs a, long long ntile
 // were binned); the stencil wraps periodically and covers an axis once at
 // most.  Contacts are buffered as working positions in LDS, sorted by tag
 // and accumulated; beyond the buffer, repeated scans take them in tag order.
+// Distance from a point to the cells of a stencil: per axis the stencil's
+// first cell (unwrapped index l) and the cell width; axes the stencil
+// covers whole (or that are not split) contribute nothing.  Used for
+// ND >= 3 only, where the corner cells of a box stencil hold a large share
+// of its volume.
+template <int ND>
+struct StencilBox {
+    double x0[ND], w[ND];
+    bool on[ND];
+    __device__ __forceinline__ void clear(int d) { on[d] = false; }
+    __device__ __forceinline__ void set(int d, double xd, long long l, double scale)
+    {
+        on[d] = ND >= 3;
+        w[d] = 1.0 / scale;
+        x0[d] = xd - (double)l * w[d];   // point relative to the first cell's lower edge
+    }
+    // squared gap along axis d to the o-th cell of the stencil
+    __device__ __forceinline__ double gap2(int d, int o) const
+    {
+        if (ND < 3 || !on[d]) return 0.0;
+        const double c0 = (double)o * w[d], c1 = c0 + w[d];
+        const double e = x0[d] < c0 ? c0 - x0[d] : (x0[d] > c1 ? x0[d] - c1 : 0.0);
+        return e * e;
+    }
+};
+
 template <int ND, class Fn>
 __device__ __forceinline__ void for_each_class_candidate(const StepArgs& a, const MultiGrid& mg,
                                                          const double* x, double D, Fn&& fn)
@@ -962,9 +988,11 @@ __device__ __forceinline__ void for_each_class_candidate(const StepArgs& a, cons
         const CellGrid& gr = mg.grid[b];
         const double R = 0.5 * (D + 2.0 * mg.thr[b] * mg.grow) + a.pad;
         int lo[ND], cnt[ND];
+        StencilBox<ND> box;
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
             const int nc = gr.nc[d];
+            box.clear(d);
             if (nc < 3) {
                 lo[d] = 0;
                 cnt[d] = 1;
@@ -981,22 +1009,28 @@ __device__ __forceinline__ void for_each_class_candidate(const StepArgs& a, cons
                 if (m < 0) m += nc;
                 lo[d] = (int)m;
                 cnt[d] = (int)(h - l + 1);
+                box.set(d, x[d], l, s);
             }
         }
+        const double R2 = R * R * (1.0 + 1e-6);
         int ncell = 1;
 #pragma unroll
         for (int d = 0; d < ND; ++d) ncell *= cnt[d];
         for (int s = 0; s < ncell; ++s) {
             int t = s;
             long long cc = 0, stride = 1;
+            double far2 = 0.0;
 #pragma unroll
             for (int d = 0; d < ND; ++d) {
-                int k = lo[d] + t % cnt[d];
+                const int o = t % cnt[d];
+                int k = lo[d] + o;
                 t /= cnt[d];
                 if (k >= gr.nc[d]) k -= gr.nc[d];
                 cc += (long long)k * stride;
                 stride *= gr.nc[d];
+                far2 += box.gap2(d, o);
             }
+            if (far2 > R2) continue;   // no point of the cell within R
             cc += mg.base[b];
             const uint32_t q0 = a.cell_start[cc], q1 = a.cell_start[cc + 1];
             for (uint32_t qj = q0; qj < q1; ++qj) fn(qj);
@@ -1135,12 +1169,14 @@ __device__ __forceinline__ void wide_particles(const StepArgs& a, const MultiGri
             const CellGrid& gr = mg.grid[b];
             const double R = 0.5 * (o.D + 2.0 * mg.thr[b] * mg.grow) + a.pad;
             int lo[ND], ct[ND];
+            StencilBox<ND> box;
             int ncell = 1;
 #pragma unroll
             for (int d = 0; d < ND; ++d) {
                 const int nc = gr.nc[d];
                 lo[d] = 0;
                 ct[d] = nc < 3 ? 1 : nc;
+                box.clear(d);
                 if (nc >= 3) {
                     const double s = gr.cell_scale[d];
                     const long long l = (long long)floor((o.x[d] - R) * s - 1e-9);
@@ -1150,21 +1186,27 @@ __device__ __forceinline__ void wide_particles(const StepArgs& a, const MultiGri
                         if (m < 0) m += nc;
                         lo[d] = (int)m;
                         ct[d] = (int)(h - l + 1);
+                        box.set(d, o.x[d], l, s);
                     }
                 }
                 ncell *= ct[d];
             }
+            const double R2 = R * R * (1.0 + 1e-6);
             for (int s = lane; s < ncell; s += kWave) {
                 int t = s;
                 long long cc = 0, stride = 1;
+                double far2 = 0.0;
 #pragma unroll
                 for (int d = 0; d < ND; ++d) {
-                    int k = lo[d] + t % ct[d];
+                    const int oc = t % ct[d];
+                    int k = lo[d] + oc;
                     t /= ct[d];
                     if (k >= gr.nc[d]) k -= gr.nc[d];
                     cc += (long long)k * stride;
                     stride *= gr.nc[d];
+                    far2 += box.gap2(d, oc);
                 }
+                if (far2 > R2) continue;
                 cc += mg.base[b];
                 const uint32_t q0 = a.cell_start[cc], q1 = a.cell_start[cc + 1];
                 for (uint32_t qj = q0; qj < q1; ++qj) {
@@ -1515,12 +1557,19 @@ __global__ __launch_bounds__(kBlock) void k_place(BinArgs a)
 // One thread per cell: order the cell's members by their previous working
 // position, which makes the new order independent of atomic arrival order.
 // Cells of up to 16 members are sorted in registers by a bitonic network
-// (padding sorts last); larger ones by insertion in memory.
-__device__ __forceinline__ void cell_order_one(const BinArgs& a, long long c)
+// (padding sorts last); larger ones are listed (item, cell) in a.big for
+// one workgroup each (k_big_cells), or without a list sorted by insertion.
+__device__ __forceinline__ void cell_order_one(const BinArgs& a, long long c, uint32_t item)
 {
     const uint32_t q0 = a.cell_start[c], q1 = a.cell_start[c + 1];
     const uint32_t n = q1 - q0;
     if (n <= 1) return;
+    if (n > 16 && a.big) {
+        const uint32_t k = atomicAdd(a.big_count, 1u);
+        a.big[2 * (size_t)k] = item;
+        a.big[2 * (size_t)k + 1] = (uint32_t)c;
+        return;
+    }
     if (n <= 16) {
         uint32_t v[16];
 #pragma unroll
@@ -1562,7 +1611,75 @@ __device__ __forceinline__ void cell_order_one(const BinArgs& a, long long c)
 __global__ __launch_bounds__(kBlock) void k_cell_order(BinArgs a)
 {
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < a.grid.total) cell_order_one(a, c);
+    if (c < a.grid.total) cell_order_one(a, c, 0u);
+}
+
+constexpr int kBigBlock = 256;
+constexpr int kBigLds = 8192;      // members sorted in LDS; larger cells by heap sort
+constexpr unsigned kBigGrid = 512; // workgroups taking the listed cells in turn
+
+// Ascending sort of perm[q0 .. q0 + n) by one workgroup: bitonic in LDS, or
+// for cells beyond kBigLds members an in-place heap sort by one thread.
+__device__ void sort_cell_members(uint32_t* perm, uint32_t q0, uint32_t n, uint32_t* lds)
+{
+    if (n > (uint32_t)kBigLds) {
+        if (threadIdx.x == 0) {
+            uint32_t* v = perm + q0;
+            auto sift = [&](uint32_t root, uint32_t end) {
+                while (2 * root + 1 < end) {
+                    uint32_t ch = 2 * root + 1;
+                    if (ch + 1 < end && v[ch] < v[ch + 1]) ++ch;
+                    if (v[root] >= v[ch]) return;
+                    const uint32_t t = v[root];
+                    v[root] = v[ch];
+                    v[ch] = t;
+                    root = ch;
+                }
+            };
+            for (uint32_t i = n / 2; i-- > 0;) sift(i, n);
+            for (uint32_t end = n - 1; end > 0; --end) {
+                const uint32_t t = v[0];
+                v[0] = v[end];
+                v[end] = t;
+                sift(0, end);
+            }
+        }
+        __syncthreads();
+        return;
+    }
+    uint32_t P = 1;
+    while (P < n) P <<= 1;
+    for (uint32_t i = threadIdx.x; i < P; i += kBigBlock) lds[i] = i < n ? perm[q0 + i] : 0xffffffffu;
+    __syncthreads();
+    for (uint32_t size = 2; size <= P; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+            for (uint32_t i = threadIdx.x; i < P; i += kBigBlock) {
+                const uint32_t l = i ^ stride;
+                if (l > i) {
+                    const uint32_t vi = lds[i], vl = lds[l];
+                    const bool up = (i & size) == 0;
+                    if ((vi > vl) == up) {
+                        lds[i] = vl;
+                        lds[l] = vi;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t i = threadIdx.x; i < n; i += kBigBlock) perm[q0 + i] = lds[i];
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kBigBlock) void k_big_cells(BinArgs a)
+{
+    __shared__ uint32_t lds[kBigLds];
+    const uint32_t count = *a.big_count;
+    for (uint32_t k = blockIdx.x; k < count; k += gridDim.x) {
+        const uint32_t c = a.big[2 * (size_t)k + 1];
+        const uint32_t q0 = a.cell_start[c];
+        sort_cell_members(a.perm, q0, a.cell_start[c + 1] - q0, lds);
+    }
 }
 
 // New position q takes old position perm[q]; anchor_o (optional) receives
@@ -2138,7 +2255,21 @@ __global__ __launch_bounds__(kBatchBinBlock) void kb_cell_order(const BatchBin* 
     const int e = table_entry(t, blockIdx.x);
     const BatchBin& B = bins[table_item(t, e)];
     const long long c = (long long)(blockIdx.x - t.prefix[e]) * kBatchBinBlock + threadIdx.x;
-    if (c < B.b.grid.total) cell_order_one(B.b, c);
+    if (c < B.b.grid.total) cell_order_one(B.b, c, (uint32_t)table_item(t, e));
+}
+
+__global__ __launch_bounds__(kBigBlock) void kb_big_cells(const BatchBin* __restrict__ bins,
+                                                          const uint32_t* __restrict__ big,
+                                                          const uint32_t* __restrict__ big_count)
+{
+    __shared__ uint32_t lds[kBigLds];
+    const uint32_t count = *big_count;
+    for (uint32_t k = blockIdx.x; k < count; k += gridDim.x) {
+        const BatchBin& B = bins[big[2 * (size_t)k]];
+        const uint32_t c = big[2 * (size_t)k + 1];
+        const uint32_t q0 = B.b.cell_start[c];
+        sort_cell_members(B.b.perm, q0, B.b.cell_start[c + 1] - q0, lds);
+    }
 }
 
 template <int ND>
@@ -2326,6 +2457,15 @@ int launch_exact_step(const StepArgs& a, uint32_t* ref_count, uint32_t* ref_star
     return check_launch();
 }
 
+// in-cell order of a binning: small cells per thread, listed big cells by
+// one workgroup each
+static void cell_order(const BinArgs& a, hipStream_t s)
+{
+    if (a.big) k_fill_u32<<<1, 1, 0, s>>>(a.big_count, 0u, 1);
+    k_cell_order<<<grid_for((size_t)a.grid.total), kBlock, 0, s>>>(a);
+    if (a.big) k_big_cells<<<kBigGrid, kBigBlock, 0, s>>>(a);
+}
+
 int launch_rebin(BinArgs& a, void* scan_tmp, hipStream_t s)
 {
     const int N = a.g.N;
@@ -2334,7 +2474,7 @@ int launch_rebin(BinArgs& a, void* scan_tmp, hipStream_t s)
     RCP_DISPATCH_ND(a.g.ndim, k_bin, <<<grid_for(N), kBlock, 0, s>>>(a));
     if (launch_scan_u32(a.cell_count, a.cell_start, total, scan_tmp, s)) return 1;
     k_place<<<grid_for(N), kBlock, 0, s>>>(a);
-    k_cell_order<<<grid_for(total), kBlock, 0, s>>>(a);
+    cell_order(a, s);
     RCP_DISPATCH_ND(a.g.ndim, k_permute, <<<grid_for(N), kBlock, 0, s>>>(a));
     return check_launch();
 }
@@ -2347,7 +2487,7 @@ int launch_rebin_multi(BinArgs& a, const MultiGrid& mg, void* scan_tmp, hipStrea
     RCP_DISPATCH_ND(a.g.ndim, k_bin_multi, <<<grid_for(N), kBlock, 0, s>>>(a, mg));
     if (launch_scan_u32(a.cell_count, a.cell_start, total, scan_tmp, s)) return 1;
     k_place<<<grid_for(N), kBlock, 0, s>>>(a);
-    k_cell_order<<<grid_for(total), kBlock, 0, s>>>(a);
+    cell_order(a, s);
     RCP_DISPATCH_ND(a.g.ndim, k_permute, <<<grid_for(N), kBlock, 0, s>>>(a));
     return check_launch();
 }
@@ -2601,14 +2741,17 @@ int launch_batch_step(int ndim, const BatchPack* packs, int npacks, const BatchT
 }
 
 int launch_batch_rebin(int ndim, const BatchBin* bins, int nbins, const BatchTable& parts,
-                       const BatchTable& cells, hipStream_t s)
+                       const BatchTable& cells, uint32_t* big, uint32_t* big_count,
+                       hipStream_t s)
 {
     if (nbins <= 0) return 0;
     kb_clear_cells<<<cells.blocks, kBatchBinBlock, 0, s>>>(bins, cells);
     RCP_DISPATCH_ND(ndim, kb_bin_multi, <<<parts.blocks, kBatchBinBlock, 0, s>>>(bins, parts));
     kb_scan_cells<<<(unsigned)nbins, kFoldBlock, 0, s>>>(bins);
     kb_place<<<parts.blocks, kBatchBinBlock, 0, s>>>(bins, parts);
+    k_fill_u32<<<1, 1, 0, s>>>(big_count, 0u, 1);
     kb_cell_order<<<cells.blocks, kBatchBinBlock, 0, s>>>(bins, cells);
+    kb_big_cells<<<kBigGrid, kBigBlock, 0, s>>>(bins, big, big_count);
     RCP_DISPATCH_ND(ndim, kb_permute, <<<parts.blocks, kBatchBinBlock, 0, s>>>(bins, parts));
     return check_launch();
 }
