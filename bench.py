@@ -53,12 +53,30 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def workload(N: int):
-    mu = -0.5 * math.log(1.04)
-    sigma = math.sqrt(math.log(1.04))
-    init = {"phi": 0.10, "N": N, "Ndim": 2, "box": [1.0, 1.0], "walls": [0, 0], "seed": 123,
-            "dist": {"type": "lognormal", "mu": mu, "sigma": sigma}}
-    return init
+def workload(N: int, config: str = "2"):
+    """Initializer input of the single-packing configurations: "2" (2-D
+    log-normal, periodic), "3" (3-D power law 10:1, hard walls in z), "3s"
+    (the same in a spherical container)."""
+    if config == "2":
+        mu = -0.5 * math.log(1.04)
+        sigma = math.sqrt(math.log(1.04))
+        return {"phi": 0.10, "N": N, "Ndim": 2, "box": [1.0, 1.0], "walls": [0, 0],
+                "seed": 123, "dist": {"type": "lognormal", "mu": mu, "sigma": sigma}}
+    walls = [0, 0, 1] if config == "3" else [-3, -1, -1]
+    return {"phi": 0.05, "N": N, "Ndim": 3, "box": [1.0, 1.0, 1.0], "walls": walls,
+            "seed": 123, "dist": {"type": "powerlaw", "d_min": 0.1, "d_max": 1.0,
+                                  "exponent": -3.0}}
+
+
+CONFIG_TEXT = {
+    "2": "plain method single large packing: 2-D, N=1,000,000 disks, periodic square box, "
+         "log-normal diameters (mean 1.0, std 0.2), initial phi 0.10, seed 123, steps 1..K",
+    "3": "plain method high polydispersity: 3-D, N=1,000,000 spheres, power law P(d) ~ d^-3 "
+         "on [0.1, 1.0], hard walls in z, periodic in x and y, initial phi 0.05, seed 123, "
+         "steps 1..K",
+    "3s": "plain method high polydispersity: 3-D, N=1,000,000 spheres, power law P(d) ~ d^-3 "
+          "on [0.1, 1.0], spherical container, initial phi 0.05, seed 123, steps 1..K",
+}
 
 
 def host_cores() -> int:
@@ -171,33 +189,9 @@ def pmc_summary(fetch_csv, write_csv):
 
 
 def batch_workload(P: int, seed0: int = 0):
-    """Config-4 generator: packing i (seed i) draws N ~ U{500..20000}, Ndim
-    from {2, 3, 4}, a diameter family (mono; bidisperse 1:1.4; log-normal
-    with cv ~ U[0.1, 0.3]; power law with ratio ~ U[2, 10], exponent -2.5),
-    periodic or hard wall per dimension and phi ~ U[0.05, 0.15]."""
-    import random
-    cases = []
-    for i in range(seed0, seed0 + P):
-        r = random.Random(i)
-        N = r.randint(500, 20000)
-        nd = r.choice((2, 3, 4))
-        fam = r.choice(("mono", "bidisperse", "lognormal", "powerlaw"))
-        if fam == "mono":
-            dist = {"type": "mono", "d": 1.0}
-        elif fam == "bidisperse":
-            dist = {"type": "bidisperse", "d1": 1.0, "d2": 1.4, "p": 0.5}
-        elif fam == "lognormal":
-            cv = r.uniform(0.1, 0.3)
-            s2 = math.log(1.0 + cv * cv)
-            dist = {"type": "lognormal", "mu": -0.5 * s2, "sigma": math.sqrt(s2)}
-        else:
-            ratio = r.uniform(2.0, 10.0)
-            dist = {"type": "powerlaw", "d_min": 1.0 / ratio, "d_max": 1.0, "exponent": -2.5}
-        walls = [r.choice((0, 1)) for _ in range(nd)]
-        phi = r.uniform(0.05, 0.15)
-        cases.append({"phi": phi, "N": N, "Ndim": nd, "box": [1.0] * nd, "walls": walls,
-                      "seed": i, "dist": dist})
-    return cases
+    """The batched configuration's generator (rcpgenerator_amd.workloads)."""
+    from rcpgenerator_amd.workloads import batch_cases
+    return batch_cases(P, seed0)
 
 
 def bench_batch(args) -> int:
@@ -213,9 +207,16 @@ def bench_batch(args) -> int:
         inits = list(ex.map(capi.initialize, cases))
     log(f"[bench] initializer {time.perf_counter() - t0:.1f}s for {len(cases)} packings, "
         f"{sum(c['N'] for c in cases)} particles")
-    runs = [(x, D, b, w, None) for (x, D, b, w) in inits]
+    opts = {"max_steps": args.max_steps} if args.max_steps else None
+    runs = [(x, D, b, w, opts) for (x, D, b, w) in inits]
     t0 = time.perf_counter()
-    res, st = capi.run_batch(runs, threads=args.threads, pool=args.pool)
+    def progress(done, total):
+        if done % max(1, total // 8) == 0 or done == total:
+            log(f"[bench] {done}/{total} packings finished at {time.perf_counter() - t0:.1f}s")
+    res, st = capi.run_batch(
+        runs, threads=args.threads, pool=args.pool, progress=progress, every=2000,
+        rounds_progress=lambda r, f, n: log(f"[bench] round {r}: {f}/{n} finished at "
+                                            f"{time.perf_counter() - t0:.1f}s"))
     dt = time.perf_counter() - t0
     bad = [r for r in res if r is None or r["rc"] != 0]
     if bad:
@@ -232,6 +233,8 @@ def bench_batch(args) -> int:
     upd = sum(c["N"] * s for c, s in zip(cases, steps))
     value = len(res) / (dt / 3600.0)
     engine = "engine pool" if args.pool else "batched engine"
+    if args.max_steps:
+        engine += f", every packing stopped at {args.max_steps} steps: not a throughput figure"
     line = {"metric": f"jammed packings/hour ({engine}, plain method)", "value": value,
             "unit": "packings/hour", "n_gpus": 1, "packings": len(res), "jammed": jammed,
             "wall_s": dt, "particle_updates_per_s": upd / dt, "steps_total": sum(steps),
@@ -318,10 +321,15 @@ def main() -> int:
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     ap.add_argument("--workload", choices=("single", "batch"), default="single")
+    ap.add_argument("--config", choices=("2", "3", "3s"), default="2",
+                    help="single-packing configuration (2: the headline 2-D workload)")
     ap.add_argument("--packings", type=int, default=64)
     ap.add_argument("--threads", type=int, default=8)
     ap.add_argument("--pool", action="store_true",
                     help="batch workload on the engine pool instead of the batched engine")
+    ap.add_argument("--max-steps", type=int, default=0,
+                    help="batch workload: stop every packing after this many steps (profiling "
+                         "runs; the packings/hour figure needs full runs)")
     args = ap.parse_args()
     if args.pmc_summary:
         print(json.dumps(pmc_summary(*args.pmc_summary), indent=1))
@@ -339,7 +347,8 @@ def main() -> int:
     N, K, W = args.n, args.steps, args.warmup
     log(f"[bench] device: {rcp._rcpgenerator.device_info()}")
     t0 = time.perf_counter()
-    init = rcp.initialize_particles(workload(N))
+    init = rcp.initialize_particles(workload(N, args.config))
+    nd = len(init["box"])
     log(f"[bench] initializer {time.perf_counter() - t0:.1f}s")
     inp = {"positions": init["positions"], "diameters": init["diameters"]}
     cfg = {"box": init["box"], "walls": init["walls"]}
@@ -366,12 +375,17 @@ def main() -> int:
     value = N * done / dt
     launches = info["step_kernel_launches"] - base["step_kernel_launches"]
     kms = (info["step_kernel_ms"] - base["step_kernel_ms"]) / max(launches, 1)
-    algo_bytes = 8 * (6 * 2 + 1) * N
+    algo_bytes = 8 * (6 * nd + 1) * N
     achieved = algo_bytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
-    kname = "k_step_tile<2,128> (fused forces + walls + reductions + ADAM)"
+    if tiles and info.get("direct_steps", 0) == base.get("direct_steps", 0) and nd == 2:
+        kname = "k_step_tile<2,128> (fused forces + walls + reductions + ADAM)"
+    else:
+        kname = (f"step launch of the size-class cells: k_step_multi<{nd}> + k_step_wide<{nd}> "
+                 f"(fused forces + walls + reductions + ADAM)")
     roofline = {"bound": "hbm", "kernel": kname,
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(),
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": read_traffic() if args.config == "2" else None,
                 "avg_launch_ms": round(kms, 5), "algorithmic_bytes_per_launch": algo_bytes,
                 "rebin_ms_per_step": round(
                     (info["build_ms"] - base["build_ms"]) / max(done, 1), 5)}
@@ -387,10 +401,8 @@ def main() -> int:
         "steps": done, "warmup": W, "ms_per_step": 1e3 * dt / max(done, 1),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "plain method single large packing: 2-D, N=1,000,000 disks, "
-                               "periodic square box, log-normal diameters (mean 1.0, std 0.2), "
-                               "initial phi 0.10, seed 123, steps 1..K",
-                   "N": N, "ndim": 2, "parallelism": "single packing, one GPU"},
+        "config": {"workload": CONFIG_TEXT[args.config],
+                   "N": N, "ndim": nd, "parallelism": "single packing, one GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -190,6 +190,13 @@ enum {
 int rcp_run_batch(const rcp_run_config* cfgs, size_t count, int threads, int flags,
                   rcp_batch_sink sink, void* user);
 
+/* Progress hook of the rcp_run_batch calls on this thread: fn(user, rounds,
+ * finished, count) after every `every` launch rounds of the batched engine
+ * (fn NULL: none). */
+typedef void (*rcp_batch_progress_fn)(void* user, uint64_t rounds, size_t finished,
+                                      size_t count);
+int rcp_batch_set_progress(rcp_batch_progress_fn fn, void* user, uint64_t every);
+
 /* Counters of the last rcp_run_batch on this thread: launch rounds of the
  * batched engine, packing-steps they advanced, and the device time of
  * their re-binning and step launches (ms). */

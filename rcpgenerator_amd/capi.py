@@ -29,7 +29,8 @@ SYMBOLS = (
     "rcp_engine_run", "rcp_engine_finish", "rcp_engine_sync", "rcp_engine_info",
     "rcp_engine_result", "rcp_engine_trace", "rcp_engine_stats", "rcp_engine_timing",
     "rcp_engine_destroy",
-    "rcp_run_packing_observed", "rcp_run_batch", "rcp_batch_last_stats", "rcp_forces", "rcp_pairs", "rcp_mean_force",
+    "rcp_run_packing_observed", "rcp_run_batch", "rcp_batch_last_stats", "rcp_batch_set_progress",
+    "rcp_forces", "rcp_pairs", "rcp_mean_force",
     "rcp_adam_positions", "rcp_reorder_perm", "rcp_scale_diameters", "rcp_init_config_init",
     "rcp_initialize_particles", "rcp_mlp_create", "rcp_mlp_destroy", "rcp_mlp_num_params",
     "rcp_mlp_get_params", "rcp_mlp_set_params", "rcp_mlp_predict", "rcp_mlp_gradients",
@@ -79,6 +80,7 @@ class InitConfig(C.Structure):
 
 
 BatchSink = C.CFUNCTYPE(None, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p)
+BatchProgressFn = C.CFUNCTYPE(None, C.c_void_p, C.c_uint64, C.c_size_t, C.c_size_t)
 RCP_BATCH_POOL = 1
 
 
@@ -143,6 +145,7 @@ def lib():
                                     C.c_float, C.c_uint64, C.c_void_p]
         L.rcp_run_batch.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, BatchSink,
                                     C.c_void_p]
+        L.rcp_batch_set_progress.argtypes = [BatchProgressFn, C.c_void_p, C.c_uint64]
         L.rcp_batch_last_stats.argtypes = [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                            C.POINTER(C.c_double)]
         _lib = L
@@ -315,13 +318,15 @@ class Engine:
         return out
 
 
-def run_batch(runs, threads: int = 0, pool: bool = False, diameters: bool = False):
+def run_batch(runs, threads: int = 0, pool: bool = False, diameters: bool = False,
+              progress=None, rounds_progress=None, every: int = 1000):
     """rcp_run_batch over runs = [(positions, diameters, box, walls, options)]
     without building result dicts: per packing a dict of steps, phi,
     max_min_dist, force_magnitude (and the final diameters when asked).
     options keys: max_steps, exact_reductions, neighbor_mode, rung1_deadline, mu,
-    initial_phi_target.  Returns
-    (summaries, stats)."""
+    initial_phi_target.  progress(done, total), when given, is called as
+    packings finish; rounds_progress(rounds, finished, batched) every `every`
+    launch rounds of the batched engine.  Returns (summaries, stats)."""
     keep = []
     cfgs = (RunConfig * len(runs))()
     for k, (x, D, box, walls, opts) in enumerate(runs):
@@ -346,8 +351,12 @@ def run_batch(runs, threads: int = 0, pool: bool = False, diameters: bool = Fals
             c.has_initial_phi_target = 1
             c.initial_phi_target = float(opts["initial_phi_target"])
     out = [None] * len(runs)
+    finished = [0]
 
     def sink(_user, index, rc, e):
+        finished[0] += 1
+        if progress is not None:
+            progress(finished[0], len(runs))
         if rc != RCP_OK:
             out[index] = {"rc": rc, "error": lib().rcp_last_error().decode()}
             return
@@ -362,8 +371,16 @@ def run_batch(runs, threads: int = 0, pool: bool = False, diameters: bool = Fals
         out[index] = r
 
     cb = BatchSink(sink)
-    rc = lib().rcp_run_batch(C.cast(cfgs, C.c_void_p), len(runs), int(threads),
-                             RCP_BATCH_POOL if pool else 0, cb, None)
+    pcb = None
+    if rounds_progress is not None:
+        pcb = BatchProgressFn(lambda _u, r, f, n: rounds_progress(r, f, n))
+        lib().rcp_batch_set_progress(pcb, None, int(every))
+    try:
+        rc = lib().rcp_run_batch(C.cast(cfgs, C.c_void_p), len(runs), int(threads),
+                                 RCP_BATCH_POOL if pool else 0, cb, None)
+    finally:
+        if pcb is not None:
+            lib().rcp_batch_set_progress(BatchProgressFn(), None, 0)
     rounds, steps, ms = C.c_uint64(), C.c_uint64(), C.c_double()
     lib().rcp_batch_last_stats(C.byref(rounds), C.byref(steps), C.byref(ms))
     if rc != RCP_OK and all(r is None or r.get("rc") == RCP_OK for r in out):

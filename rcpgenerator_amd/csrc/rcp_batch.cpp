@@ -92,6 +92,7 @@ public:
     void complete();
     double gpu_ms() const { return gpu_ms_; }
     uint64_t member_steps() const { return member_steps_; }
+    uint64_t launches() const { return launches_; }
 
 private:
     // device arrays (each the concatenation of the members' slices)
@@ -125,7 +126,7 @@ private:
     hipStream_t s_;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     double gpu_ms_ = 0.0;
-    uint64_t member_steps_ = 0;
+    uint64_t member_steps_ = 0, launches_ = 0;
 
     void allocate();
     void upload();
@@ -476,6 +477,7 @@ bool Group::launch(const BatchSink& done)
     }
     stepping_ = live_;
     stage_and_launch(true);
+    ++launches_;
     return true;
 }
 
@@ -511,16 +513,18 @@ bool batch_eligible(const rcp_run_config& c)
 }
 
 BatchStats run_batched(const rcp_run_config* cfgs, const std::vector<size_t>& idx,
-                       const BatchSink& sink)
+                       const BatchSink& sink, const BatchProgress& progress)
 {
     BatchStats st;
     // every packing is reported exactly once, also when a group fails
     std::vector<size_t> pending(idx);
     std::sort(pending.begin(), pending.end());
     std::vector<char> reported(pending.size(), 0);
+    std::size_t finished = 0;
     const BatchSink done = [&](std::size_t i, int rc, std::unique_ptr<Engine> e) {
         const auto it = std::lower_bound(pending.begin(), pending.end(), i);
         reported[it - pending.begin()] = 1;
+        ++finished;
         sink(i, rc, std::move(e));
     };
     // controllers (set-up errors are reported per packing)
@@ -554,14 +558,30 @@ BatchStats run_batched(const rcp_run_config* cfgs, const std::vector<size_t>& id
             groups.emplace_back(new Group(d, std::move(by_dim[d]), s));
         }
         for (auto& g : groups) g->initial(done);
-        bool any = true;
+        // Each group completes its step and launches the next one while the
+        // other groups' launches run, so host work overlaps device work.
+        auto rounds = [&]() {
+            uint64_t r = 0;
+            for (auto& g : groups) r = std::max(r, g->launches());
+            return r;
+        };
+        bool any = false;
+        for (auto& g : groups)
+            if (g->active() && g->launch(done)) any = true;
+        uint64_t reported_at = 0;
         while (any) {
             any = false;
-            for (auto& g : groups)
+            for (auto& g : groups) {
+                g->complete();
                 if (g->active() && g->launch(done)) any = true;
-            for (auto& g : groups) g->complete();
-            if (any) ++st.steps;
+            }
+            st.steps = rounds();
+            if (progress.fn && progress.every && st.steps >= reported_at + progress.every) {
+                reported_at = st.steps - st.steps % progress.every;
+                progress.fn(st.steps, finished);
+            }
         }
+        st.steps = rounds();
     } catch (const HipFailure& ex) {
         g_last_error = ex.what();
         rc = RCP_E_HIP;

@@ -25,9 +25,15 @@ struct BatchStats {
 // no frozen diameters, fix_height or target_phi.
 bool batch_eligible(const rcp_run_config& c);
 
+// Called every `every` launch rounds with (rounds, packings finished).
+struct BatchProgress {
+    std::function<void(uint64_t, std::size_t)> fn;
+    uint64_t every = 0;
+};
+
 // Runs the packings cfgs[i], i in idx, to completion on the current device,
 // grouped by dimension; `done` is called on this thread for each.
 BatchStats run_batched(const rcp_run_config* cfgs, const std::vector<size_t>& idx,
-                       const BatchSink& done);
+                       const BatchSink& done, const BatchProgress& progress = BatchProgress());
 
 }  // namespace rcp

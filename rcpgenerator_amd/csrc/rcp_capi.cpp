@@ -25,6 +25,9 @@ struct rcp_engine {
 namespace {
 
 thread_local rcp::BatchStats g_batch_stats;
+thread_local rcp_batch_progress_fn g_progress_fn = nullptr;
+thread_local void* g_progress_user = nullptr;
+thread_local uint64_t g_progress_every = 0;
 
 template <class F>
 int guarded(F&& f)
@@ -362,11 +365,27 @@ int rcp_run_batch(const rcp_run_config* cfgs, size_t count, int threads, int fla
                     sink(user, i, rc, nullptr);
                 }
             };
-            g_batch_stats = rcp::run_batched(cfgs, batched, done);
+            rcp::BatchProgress prog;
+            if (g_progress_fn && g_progress_every) {
+                const rcp_batch_progress_fn fn = g_progress_fn;
+                void* pu = g_progress_user;
+                const size_t n = batched.size();
+                prog.fn = [fn, pu, n](uint64_t r, size_t f) { fn(pu, r, f, n); };
+                prog.every = g_progress_every;
+            }
+            g_batch_stats = rcp::run_batched(cfgs, batched, done, prog);
         }
         for (auto& th : pool) th.join();
         return first_rc.load();
     });
+}
+
+int rcp_batch_set_progress(rcp_batch_progress_fn fn, void* user, uint64_t every)
+{
+    g_progress_fn = fn;
+    g_progress_user = user;
+    g_progress_every = fn ? every : 0;
+    return RCP_OK;
 }
 
 int rcp_batch_last_stats(uint64_t* rounds, uint64_t* packing_steps, double* device_ms)

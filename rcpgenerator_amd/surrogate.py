@@ -3,18 +3,19 @@
 A small network (``capi.Mlp``: 3 hidden layers of 128, FP32, trained on the
 GPU by this library's own forward and backward kernels) maps the diameter
 distribution's moments, the size ratio, the dimension and the boundary to
-two numbers: the jamming fraction phi_J the plain method reaches, and the
-fraction of the default rung-1 deadline a warm-started run needs.  A warm
-start (``warm_options``) begins the run just below the predicted phi_J
-(``initial_phi_target``) with the shortened rung-1 deadline
-(``rung1_deadline``, the reference's RCP_RUNG1_DEADLINE hook,
+two numbers: the jamming fraction phi_J the plain method reaches, and a
+score for whether a warm start with a short schedule keeps that phi_J.  A
+warm start (``warm_options``) begins the run WARM_DELTA below the predicted
+phi_J (``initial_phi_target``) with rung 1 cut to SHORT_FRAC of its preset
+deadline (``rung1_deadline``, the reference's RCP_RUNG1_DEADLINE hook,
 rcp_generator.cpp:5864-5868).  The plain growth and minimisation still
-finish every packing; ``run_warm_batch`` judges each finished packing
-against the prediction and reruns it with the default schedule when the
+finish every packing; ``evaluate`` judges each warm-started packing against
+the prediction and runs it again with the default schedule when the
 prediction missed.
 
 Training rows are census rows (``rcpgenerator_amd.census``) with the
-measured phi_J and the deadline-fraction label.
+measured phi_J and the label ``safe`` (``campaign``).  Command line:
+``python -m rcpgenerator_amd.surrogate campaign | train | evaluate``.
 """
 from __future__ import annotations
 
@@ -106,3 +107,172 @@ class Surrogate:
         s.x_mean, s.x_std = d["x_mean"], d["x_std"]
         s.y_mean, s.y_std = d["y_mean"], d["y_std"]
         return s
+
+
+# ------------------------------------------------------------ warm starts
+
+WARM_DELTA = 0.04    # a warm start begins this far below the predicted phi_J
+SHORT_FRAC = 0.2     # and shortens rung 1 to this fraction of the preset deadline
+TOL = 0.002          # phi agreement with the plain method a warm start must keep
+
+
+def warm_options(phi_pred: float, N: int, ndim: int, frac: float = SHORT_FRAC,
+                 delta: float = WARM_DELTA) -> dict:
+    """Run options of a warm start from a predicted phi_J."""
+    return {"initial_phi_target": float(phi_pred) - delta,
+            "rung1_deadline": max(101, int(frac * default_deadline(N, ndim)))}
+
+
+def _initialize(cases, threads: int):
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max(1, threads)) as ex:
+        return list(ex.map(capi.initialize, cases))
+
+
+def _batch(inits, options, threads: int = 8):
+    import time
+    runs = [(x, D, b, w, o) for (x, D, b, w), o in zip(inits, options)]
+    t0 = time.perf_counter()
+    res, st = capi.run_batch(
+        runs, threads=threads, every=2000,
+        rounds_progress=lambda r, f, n: print(f"[surrogate] round {r}: {f}/{n} finished, "
+                                              f"{time.perf_counter() - t0:.0f}s", flush=True))
+    wall = time.perf_counter() - t0
+    bad = [k for k, r in enumerate(res) if r is None or r["rc"] != 0]
+    if bad:
+        raise RuntimeError(f"packings {bad[:5]} failed: {res[bad[0]]}")
+    return res, wall, st
+
+
+def campaign(cases, threads: int = 8, log=print) -> list:
+    """Training rows: each packing runs with the plain method (phi_J, steps)
+    and once warm-started from its own phi_J with the short schedule; the
+    label `safe` says whether that warm start kept phi within TOL."""
+    from .census import census_row
+    inits = _initialize(cases, threads)
+    plain, wall_p, _ = _batch(inits, [None] * len(inits), threads)
+    log(f"[surrogate] plain batch: {len(cases)} packings in {wall_p:.1f}s")
+    warm, wall_w, _ = _batch(inits, [warm_options(r["phi"], c["N"], c["Ndim"])
+                                     for r, c in zip(plain, cases)], threads)
+    log(f"[surrogate] warm batch: {wall_w:.1f}s")
+    rows = []
+    for c, (x, D, b, w), p, q in zip(cases, inits, plain, warm):
+        row = census_row(diameters=D, ndim=c["Ndim"], walls=list(w), result=p,
+                         search_name="surrogate-campaign", model_name=c["dist"]["type"],
+                         candidate=c["seed"], seed=c["seed"])
+        row.update({"phi_warm": q["phi"], "steps_warm": q["steps"],
+                    "safe": bool(abs(q["phi"] - p["phi"]) <= TOL)})
+        rows.append(row)
+    return rows
+
+
+def training_arrays(rows):
+    X = np.stack([features_of_row(r) for r in rows])
+    Y = np.array([[r["phi"], 1.0 if r["safe"] else 0.0] for r in rows], dtype=np.float32)
+    return X, Y
+
+
+def evaluate(surr: "Surrogate", cases, threads: int = 8, log=print) -> dict:
+    """Plain batched runs against surrogate-guided ones on the same packings.
+    Guided: packings the network calls safe start warm with the short
+    schedule, the others plain; a warm run whose phi misses the prediction
+    by more than 2 TOL is run again with the plain method."""
+    inits = _initialize(cases, threads)
+    plain, wall_p, _ = _batch(inits, [None] * len(inits), threads)
+    X = np.stack([features_of_packing(D, c["Ndim"], list(w)) for c, (x, D, b, w) in
+                  zip(cases, inits)])
+    pred = surr.predict(X)
+    warm = pred[:, 1] >= 0.5
+    opts = [warm_options(p[0], c["N"], c["Ndim"]) if wm else None
+            for p, c, wm in zip(pred, cases, warm)]
+    guided, wall_g, _ = _batch(inits, opts, threads)
+    miss = [k for k in range(len(cases))
+            if warm[k] and abs(guided[k]["phi"] - pred[k, 0]) > 2 * TOL]
+    wall_r = 0.0
+    if miss:
+        again, wall_r, _ = _batch([inits[k] for k in miss], [None] * len(miss), threads)
+        for k, r in zip(miss, again):
+            guided[k] = dict(r, steps=r["steps"] + guided[k]["steps"])
+    phi_p = np.array([r["phi"] for r in plain])
+    phi_g = np.array([r["phi"] for r in guided])
+    steps_p = sum(r["steps"] for r in plain)
+    steps_g = sum(r["steps"] for r in guided)
+    err = np.abs(pred[:, 0] - phi_p)
+    out = {
+        "packings": len(cases), "warm": int(warm.sum()), "reruns": len(miss),
+        "wall_plain_s": wall_p, "wall_guided_s": wall_g + wall_r,
+        "steps_plain": int(steps_p), "steps_guided": int(steps_g),
+        "speedup_steps": steps_p / steps_g, "speedup_wall": wall_p / (wall_g + wall_r),
+        "phi_within_tol": float(np.mean(np.abs(phi_g - phi_p) <= TOL)),
+        "phi_max_abs_diff": float(np.max(np.abs(phi_g - phi_p))),
+        "pred_mae": float(err.mean()), "pred_max_err": float(err.max()),
+        "jammed_guided": float(np.mean([r["max_min_dist"] < 5e-4 for r in guided])),
+    }
+    log(f"[surrogate] evaluation: {out}")
+    return out
+
+
+def _main(argv=None) -> int:
+    import argparse
+    import json
+    from . import census
+    from .workloads import batch_cases
+    ap = argparse.ArgumentParser(prog="python -m rcpgenerator_amd.surrogate")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    c = sub.add_parser("campaign", help="run packings and write labelled census rows")
+    c.add_argument("--packings", type=int, default=512)
+    c.add_argument("--seed0", type=int, default=200000)
+    c.add_argument("--nmin", type=int, default=500)
+    c.add_argument("--nmax", type=int, default=2000)
+    c.add_argument("--dims", default="2,3,4,5")
+    c.add_argument("--chunk", type=int, default=192, help="packings per batch")
+    c.add_argument("--out", required=True)
+    t = sub.add_parser("train", help="train the network on census rows")
+    t.add_argument("--data", required=True, nargs="+")
+    t.add_argument("--out", required=True)
+    t.add_argument("--epochs", type=int, default=400)
+    e = sub.add_parser("evaluate", help="plain vs guided batched runs on held-out packings")
+    e.add_argument("--model", required=True)
+    e.add_argument("--packings", type=int, default=128)
+    e.add_argument("--seed0", type=int, default=900000)
+    e.add_argument("--nmin", type=int, default=500)
+    e.add_argument("--nmax", type=int, default=2000)
+    e.add_argument("--dims", default="2,3,4,5")
+    e.add_argument("--out")
+    a = ap.parse_args(argv)
+    if a.cmd == "campaign":
+        dims = tuple(int(v) for v in a.dims.split(","))
+        cases = batch_cases(a.packings, a.seed0, a.nmin, a.nmax, dims)
+        total = safe = 0
+        for k in range(0, len(cases), a.chunk):
+            rows = campaign(cases[k:k + a.chunk], log=lambda m: print(m, flush=True))
+            census.append_rows(a.out, rows)
+            total += len(rows)
+            safe += sum(r["safe"] for r in rows)
+            print(json.dumps({"rows": total, "safe": safe}), flush=True)
+    elif a.cmd == "train":
+        rows = [r for p in a.data for r in census.load_rows(p)]
+        X, Y = training_arrays(rows)
+        s = Surrogate(seed=1)
+        losses = s.fit(X, Y, epochs=a.epochs)
+        s.save(a.out)
+        P = s.predict(X)
+        print(json.dumps({"rows": len(rows), "loss_first": float(losses[0]),
+                          "loss_last": float(losses[-1]),
+                          "train_phi_mae": float(np.mean(np.abs(P[:, 0] - Y[:, 0]))),
+                          "train_safe_acc": float(np.mean((P[:, 1] >= 0.5) == (Y[:, 1] >= 0.5)))}),
+              flush=True)
+    else:
+        dims = tuple(int(v) for v in a.dims.split(","))
+        out = evaluate(Surrogate.load(a.model), batch_cases(a.packings, a.seed0, a.nmin, a.nmax,
+                                                            dims),
+                       log=lambda m: print(m, flush=True))
+        if a.out:
+            with open(a.out, "w") as f:
+                json.dump(out, f)
+        print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(_main())
