@@ -69,7 +69,10 @@ def features_of_row(row: dict) -> np.ndarray:
 
 
 class Surrogate:
-    """The network plus its input / output standardisation."""
+    """The network plus its standardisation.  The phi_J output is learned as
+    the residual from the mean phi_J of the packing's dimension in the
+    training rows (the network models what the distribution and the
+    boundary change within a dimension)."""
 
     def __init__(self, seed: int = 0):
         self.net = capi.Mlp(N_FEATURES, HIDDEN, LAYERS, N_TARGETS, seed=seed)
@@ -77,27 +80,67 @@ class Surrogate:
         self.x_std = np.ones(N_FEATURES, dtype=np.float32)
         self.y_mean = np.zeros(N_TARGETS, dtype=np.float32)
         self.y_std = np.ones(N_TARGETS, dtype=np.float32)
+        self.dim_phi = np.zeros(9, dtype=np.float32)   # mean phi_J per dimension
+        self.val_loss = None
 
-    def fit(self, X, Y, epochs: int = 300, batch: int = 128, lr: float = 1e-3,
-            seed: int = 0) -> np.ndarray:
+    @staticmethod
+    def _dims(X) -> np.ndarray:
+        return np.rint(np.asarray(X)[:, 11] * 5.0).astype(np.int64)
+
+    def _base(self, X) -> np.ndarray:
+        return self.dim_phi[np.clip(self._dims(X), 0, 8)]
+
+    def fit(self, X, Y, epochs: int = 300, batch: int = 0, lr: float = 1e-3, seed: int = 0,
+            val_frac: float = 0.0, check_every: int = 10) -> np.ndarray:
+        """Adam on the GPU (batch 0: the whole training set per step).  With
+        val_frac > 0 a seeded share of the rows is held out and the
+        parameters with the lowest held-out phi_J error (checked every
+        `check_every` epochs) are kept.  Returns the per-epoch losses."""
         X = np.asarray(X, dtype=np.float32)
-        Y = np.asarray(Y, dtype=np.float32)
-        self.x_mean = X.mean(axis=0)
-        self.x_std = np.where(X.std(axis=0) > 1e-6, X.std(axis=0), 1.0).astype(np.float32)
-        self.y_mean = Y.mean(axis=0)
-        self.y_std = np.where(Y.std(axis=0) > 1e-6, Y.std(axis=0), 1.0).astype(np.float32)
+        Y = np.array(Y, dtype=np.float32)
+        order = np.random.default_rng(seed).permutation(len(X))
+        nv = int(round(val_frac * len(X))) if len(X) >= 10 else 0
+        val, tr = order[:nv], order[nv:]
+        dims = self._dims(X)
+        self.dim_phi = np.zeros(9, dtype=np.float32)
+        for k in np.unique(dims[tr]):
+            self.dim_phi[k] = Y[tr][dims[tr] == k, 0].mean()
+        R = Y.copy()
+        R[:, 0] -= self._base(X)
+        self.x_mean = X[tr].mean(axis=0)
+        sd = X[tr].std(axis=0)
+        self.x_std = np.where(sd > 1e-6, sd, 1.0).astype(np.float32)
+        self.y_mean = R[tr].mean(axis=0)
+        sd = R[tr].std(axis=0)
+        self.y_std = np.where(sd > 1e-6, sd, 1.0).astype(np.float32)
         Xn = (X - self.x_mean) / self.x_std
-        Yn = (Y - self.y_mean) / self.y_std
-        return self.net.train(Xn, Yn, batch=batch, epochs=epochs, lr=lr, seed=seed)
+        Rn = (R - self.y_mean) / self.y_std
+        bs = batch if batch > 0 else len(tr)
+        losses, best, best_p = [], np.inf, None
+        step = check_every if nv else epochs
+        for k in range(0, epochs, step):
+            n = min(step, epochs - k)
+            losses.extend(self.net.train(Xn[tr], Rn[tr], batch=bs, epochs=n, lr=lr,
+                                         seed=seed + k))
+            if nv:
+                err = float(np.mean(np.abs(self.predict(X[val])[:, 0] - Y[val, 0])))
+                if err < best:
+                    best, best_p = err, self.net.get_params()
+        if best_p is not None:
+            self.net.set_params(best_p)
+        self.val_loss = best if nv else None
+        return np.asarray(losses, dtype=np.float32)
 
     def predict(self, X) -> np.ndarray:
+        """Rows of (phi_J, safe score) for feature rows X."""
         X = np.asarray(X, dtype=np.float32).reshape(-1, N_FEATURES)
-        Yn = self.net.predict((X - self.x_mean) / self.x_std)
-        return Yn * self.y_std + self.y_mean
+        P = self.net.predict((X - self.x_mean) / self.x_std) * self.y_std + self.y_mean
+        P[:, 0] += self._base(X)
+        return P
 
     def save(self, path) -> None:
         np.savez(path, params=self.net.get_params(), x_mean=self.x_mean, x_std=self.x_std,
-                 y_mean=self.y_mean, y_std=self.y_std)
+                 y_mean=self.y_mean, y_std=self.y_std, dim_phi=self.dim_phi)
 
     @classmethod
     def load(cls, path) -> "Surrogate":
@@ -106,10 +149,15 @@ class Surrogate:
         s.net.set_params(d["params"])
         s.x_mean, s.x_std = d["x_mean"], d["x_std"]
         s.y_mean, s.y_std = d["y_mean"], d["y_std"]
+        s.dim_phi = d["dim_phi"]
         return s
 
 
 # ------------------------------------------------------------ warm starts
+
+# mean phi_J of the plain method per dimension over the committed training
+# rows (config-4 families, N 500..2000): the predictor without the network
+DIM_MEAN_PHI = {2: 0.8519, 3: 0.6292, 4: 0.4141}
 
 WARM_DELTA = 0.04    # a warm start begins this far below the predicted phi_J
 SHORT_FRAC = 0.2     # and shortens rung 1 to this fraction of the preset deadline
@@ -172,7 +220,7 @@ def training_arrays(rows):
     return X, Y
 
 
-def evaluate(surr: "Surrogate", cases, threads: int = 8, log=print) -> dict:
+def evaluate(surr: "Surrogate", cases, threads: int = 8, log=print, detail: bool = False) -> dict:
     """Plain batched runs against surrogate-guided ones on the same packings.
     Guided: packings the network calls safe start warm with the short
     schedule, the others plain; a warm run whose phi misses the prediction
@@ -195,6 +243,8 @@ def evaluate(surr: "Surrogate", cases, threads: int = 8, log=print) -> dict:
             guided[k] = dict(r, steps=r["steps"] + guided[k]["steps"])
     phi_p = np.array([r["phi"] for r in plain])
     phi_g = np.array([r["phi"] for r in guided])
+    # the predictor without the network: the mean training phi_J of the dimension
+    base = np.array([DIM_MEAN_PHI.get(int(c["Ndim"]), float(phi_p.mean())) for c in cases])
     steps_p = sum(r["steps"] for r in plain)
     steps_g = sum(r["steps"] for r in guided)
     err = np.abs(pred[:, 0] - phi_p)
@@ -206,9 +256,16 @@ def evaluate(surr: "Surrogate", cases, threads: int = 8, log=print) -> dict:
         "phi_within_tol": float(np.mean(np.abs(phi_g - phi_p) <= TOL)),
         "phi_max_abs_diff": float(np.max(np.abs(phi_g - phi_p))),
         "pred_mae": float(err.mean()), "pred_max_err": float(err.max()),
+        "dim_mean_mae": float(np.abs(base - phi_p).mean()),
         "jammed_guided": float(np.mean([r["max_min_dist"] < 5e-4 for r in guided])),
     }
     log(f"[surrogate] evaluation: {out}")
+    if detail:
+        out.update({"warm_ids": [int(k) for k in np.nonzero(warm)[0]], "rerun_ids": miss,
+                    "steps_plain_each": [r["steps"] for r in plain],
+                    "steps_guided_each": [r["steps"] for r in guided],
+                    "phi_plain_each": phi_p.tolist(), "phi_guided_each": phi_g.tolist(),
+                    "pred_each": pred.tolist()})
     return out
 
 
@@ -230,7 +287,7 @@ def _main(argv=None) -> int:
     t = sub.add_parser("train", help="train the network on census rows")
     t.add_argument("--data", required=True, nargs="+")
     t.add_argument("--out", required=True)
-    t.add_argument("--epochs", type=int, default=400)
+    t.add_argument("--epochs", type=int, default=300)
     e = sub.add_parser("evaluate", help="plain vs guided batched runs on held-out packings")
     e.add_argument("--model", required=True)
     e.add_argument("--packings", type=int, default=128)
@@ -251,14 +308,24 @@ def _main(argv=None) -> int:
             safe += sum(r["safe"] for r in rows)
             print(json.dumps({"rows": total, "safe": safe}), flush=True)
     elif a.cmd == "train":
-        rows = [r for p in a.data for r in census.load_rows(p)]
-        X, Y = training_arrays(rows)
+        Xs, Ys = [], []
+        for p in a.data:
+            if p.endswith(".npz"):
+                d = np.load(p, allow_pickle=False)
+                Xs.append(d["X"])
+                Ys.append(d["Y"])
+            else:
+                X1, Y1 = training_arrays(census.load_rows(p))
+                Xs.append(X1)
+                Ys.append(Y1)
+        X, Y = np.concatenate(Xs), np.concatenate(Ys)
+        rows = X
         s = Surrogate(seed=1)
         losses = s.fit(X, Y, epochs=a.epochs)
         s.save(a.out)
         P = s.predict(X)
         print(json.dumps({"rows": len(rows), "loss_first": float(losses[0]),
-                          "loss_last": float(losses[-1]),
+                          "loss_last": float(losses[-1]), "val_loss": s.val_loss,
                           "train_phi_mae": float(np.mean(np.abs(P[:, 0] - Y[:, 0]))),
                           "train_safe_acc": float(np.mean((P[:, 1] >= 0.5) == (Y[:, 1] >= 0.5)))}),
               flush=True)
