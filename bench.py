@@ -290,8 +290,9 @@ def cpu_baseline_batch(inputs, configs, steps, budget_s):
     K = 20
     with mp.get_context("spawn").Pool(workers) as pool:
         per = pool.map(_ref_steps_time, [(i, c, K, native) for i, c in zip(inputs, configs)])
-    cost = sum(s * t for s, t in zip(steps, per))
-    return {"value": 3600.0 * workers / cost, "unit": "packings/hour", "cores": workers,
+    cost = sum(s * t for s, t in zip(steps, per))   # CPU-seconds of the whole batch
+    return {"value": 3600.0 * workers * len(steps) / cost, "unit": "packings/hour",
+            "cores": workers,
             "kind": "reference",
             "sample": f"each packing run {K} steps on the reference ({'AVX-512' if native else 'portable'} "
                       f"build, 1 thread per process, {workers} processes); full runs costed at "
@@ -313,7 +314,7 @@ def read_traffic():
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")

@@ -71,6 +71,11 @@ typedef struct rcp_run_config {
                                      (rcp_generator.cpp:5864-5868, 4727-4730) as a per-run
                                      value: 100 < v < 1e7 replaces the preset deadline of
                                      rung 1 (and the step cap's rung-1 term); 0 = preset */
+    size_t osc_cycles;            /* RCP_OSC_CYCLES (rcp_generator.cpp:5275-5288): 1..99
+                                     oscillation cycles; 0 = preset (2) */
+    double osc_period_factor;     /* RCP_OSC_PERIOD_FACTOR (rcp_generator.cpp:5289-5294):
+                                     period = factor * rung-1 end step, 0 < f < 100;
+                                     0 = preset (0.5) */
 } rcp_run_config;
 
 /* Fills the reference defaults (reorder_interval 2000, everything else 0). */

@@ -55,6 +55,8 @@ class RunConfig(C.Structure):
         ("time_kernels", C.c_int),
         ("neighbor_mode", C.c_int),
         ("rung1_deadline", C.c_size_t),
+        ("osc_cycles", C.c_size_t),
+        ("osc_period_factor", C.c_double),
     ]
 
 
@@ -323,8 +325,8 @@ def run_batch(runs, threads: int = 0, pool: bool = False, diameters: bool = Fals
     """rcp_run_batch over runs = [(positions, diameters, box, walls, options)]
     without building result dicts: per packing a dict of steps, phi,
     max_min_dist, force_magnitude (and the final diameters when asked).
-    options keys: max_steps, exact_reductions, neighbor_mode, rung1_deadline, mu,
-    initial_phi_target.  progress(done, total), when given, is called as
+    options keys: max_steps, exact_reductions, neighbor_mode, rung1_deadline,
+    osc_cycles, osc_period_factor, mu, initial_phi_target.  progress(done, total), when given, is called as
     packings finish; rounds_progress(rounds, finished, batched) every `every`
     launch rounds of the batched engine.  Returns (summaries, stats)."""
     keep = []
@@ -345,6 +347,8 @@ def run_batch(runs, threads: int = 0, pool: bool = False, diameters: bool = Fals
         c.exact_reductions = int(bool(opts.get("exact_reductions", False)))
         c.neighbor_mode = int(opts.get("neighbor_mode", 0))
         c.rung1_deadline = int(opts.get("rung1_deadline", 0))
+        c.osc_cycles = int(opts.get("osc_cycles", 0))
+        c.osc_period_factor = float(opts.get("osc_period_factor", 0.0))
         if "mu" in opts:
             c.has_mu, c.mu = 1, float(opts["mu"])
         if "initial_phi_target" in opts:

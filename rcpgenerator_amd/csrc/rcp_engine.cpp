@@ -526,11 +526,16 @@ void Engine::setup(const rcp_run_config& c, bool hosted)
 
     // step cap (RG:4713-4737), speed preset "quick"
     const bool deadline_set = c.rung1_deadline > 100 && c.rung1_deadline < 10000000;
+    // oscillation cycles and period factor (RG:5275-5294; the RCP_OSC_CYCLES and
+    // RCP_OSC_PERIOD_FACTOR hooks as per-run values)
+    osc_cycles_ = (c.osc_cycles >= 1 && c.osc_cycles < 100) ? c.osc_cycles : 2;
+    osc_period_factor_ = (c.osc_period_factor > 0.0 && c.osc_period_factor < 100.0)
+        ? c.osc_period_factor : 0.5;
     step_cap_ = c.max_steps > 0 ? c.max_steps : [&]() {
         double rung1 = 2.0 * 4000.0 + 9.0 * std::sqrt(double(N_) * double(nd_)) - 2000.0;
         if (deadline_set) rung1 = static_cast<double>(c.rung1_deadline);
         if (rung1 < 1500.0) rung1 = 1500.0;
-        const std::size_t formula = static_cast<std::size_t>(rung1 * double(2 + 3));
+        const std::size_t formula = static_cast<std::size_t>(rung1 * double(osc_cycles_ + 3));
         return std::max<std::size_t>(60000, formula);
     }();
     const std::size_t hist = step_cap_ + 1;
@@ -1017,7 +1022,7 @@ bool Engine::schedule_top(std::size_t step)
         osc_t_relax_ = step - mu_change_;
         if (osc_t_relax_ < 250) osc_t_relax_ = 250;
         osc_start_ = step;
-        osc_period_ = static_cast<std::size_t>(0.5 * mu_change_);
+        osc_period_ = static_cast<std::size_t>(osc_period_factor_ * mu_change_);
         if (osc_period_ < 1000) osc_period_ = 1000;
         osc_ = Osc::Active;
         mu_flag_ = -1;
@@ -1029,7 +1034,8 @@ bool Engine::schedule_top(std::size_t step)
     if (osc_ == Osc::Active) {
         const double t_into = static_cast<double>(step - osc_start_);
         const double cp = t_into / static_cast<double>(osc_period_);
-        if (cp >= 2.0) {
+        const double cycles = static_cast<double>(osc_cycles_);
+        if (cp >= cycles) {
             mu_ = osc_mu_m1_;
             osc_ = Osc::Final;
             osc_final_start_ = step;
@@ -1040,7 +1046,7 @@ bool Engine::schedule_top(std::size_t step)
             const double mu_ref = (osc_mu_0_ - osc_mu_m1_) *
                 std::exp(-t_into / static_cast<double>(osc_t_relax_)) + osc_mu_m1_;
             mu_ = mu_ref * std::pow(10.0, sin_val);
-            if (cp >= 2.0 - 0.25) {
+            if (cp >= cycles - 0.25) {
                 const double F_now = F_h_[step - 1];
                 if (F_now < 5e-3 && max_min_dist_ < 5e-4) {
                     if (++osc_consec_ >= 3) osc_ = Osc::Done;

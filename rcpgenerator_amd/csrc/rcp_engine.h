@@ -269,6 +269,8 @@ private:
     double osc_mu_0_ = 0.0, osc_mu_m1_ = 0.0;
     std::size_t osc_start_ = 0, osc_period_ = 0, osc_t_relax_ = 0, osc_final_start_ = 0,
                 osc_consec_ = 0;
+    std::size_t osc_cycles_ = 2;
+    double osc_period_factor_ = 0.5;
     double D0_min_ = 0.0, S_D0_ = 0.0, D0max_free_ = 0.0, D0max_fixed_ = 0.0, shell0_ = 0.0;
     // force-evaluation outputs
     double U_ = 0.0, Lc_ = 0.0, Fmean_ = 0.0, max_min_dist_ = 0.0, mean_force_ = 0.0;

@@ -133,6 +133,12 @@ void parse_run(const py::dict& input, const py::dict& config, const py::dict& op
     take(options, "max_steps", max_steps);
     take(options, "rung1_deadline", rung1_deadline);
     c.rung1_deadline = rung1_deadline;
+    size_t osc_cycles = 0;
+    double osc_period_factor = 0.0;
+    take(options, "osc_cycles", osc_cycles);
+    take(options, "osc_period_factor", osc_period_factor);
+    c.osc_cycles = osc_cycles;
+    c.osc_period_factor = osc_period_factor;
     take(options, "fix_diameter", fix_diameter);
     take(options, "reorder_interval", reorder_interval);
     c.max_steps = max_steps;
