@@ -343,20 +343,29 @@ __device__ __forceinline__ void particle_tail(const StepArgs& a, int q, const Ow
     }
     if (a.do_adam) {
         double d2a = 0.0, d2s = 0.0;
+        // every load before the first store: the arrays may alias as far as
+        // the compiler knows, so interleaving would serialise the round trips
+        double mm[ND], vv[ND], xa[ND];
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
             const size_t idx = (size_t)q * ND + d;
-            double mm = a.m[idx], vv = a.v[idx];
-            const double xv = adam_dof(F[d], mm, vv, o.x[d], a.g.box[d], a.alpha, a.inv_b1,
-                                       a.inv_b2);
-            a.m[idx] = mm;
-            a.v[idx] = vv;
+            mm[d] = a.m[idx];
+            vv[d] = a.v[idx];
+            xa[d] = a.x_anchor ? a.x_anchor[idx] : 0.0;
+        }
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            const size_t idx = (size_t)q * ND + d;
+            const double xv = adam_dof(F[d], mm[d], vv[d], o.x[d], a.g.box[d], a.alpha,
+                                       a.inv_b1, a.inv_b2);
+            a.m[idx] = mm[d];
+            a.v[idx] = vv[d];
             a.x_out[idx] = xv;
             double dd = xv - o.x[d];
             dd = mic(dd, a.g.inv_box[d], a.g.box[d]);
             d2s += dd * dd;
             if (a.x_anchor) {
-                double da = xv - a.x_anchor[idx];
+                double da = xv - xa[d];
                 da = mic(da, a.g.inv_box[d], a.g.box[d]);
                 d2a += da * da;
             }
@@ -1688,17 +1697,27 @@ template <int ND>
 __device__ __forceinline__ void permute_one(const BinArgs& a, int q, double* anchor_o)
 {
     const uint32_t w = a.perm[q];
+    // all loads first (one round trip), then the stores
+    double xv[ND], mv[ND], vv[ND];
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
-        const double xv = a.x[(size_t)w * ND + d];
-        a.x_o[(size_t)q * ND + d] = xv;
-        if (anchor_o) anchor_o[(size_t)q * ND + d] = xv;
-        a.m_o[(size_t)q * ND + d] = a.m[(size_t)w * ND + d];
-        a.v_o[(size_t)q * ND + d] = a.v[(size_t)w * ND + d];
+        xv[d] = a.x[(size_t)w * ND + d];
+        mv[d] = a.m[(size_t)w * ND + d];
+        vv[d] = a.v[(size_t)w * ND + d];
     }
-    a.D0_o[q] = a.D0[w];
-    a.tag_o[q] = a.tag[w];
-    if (a.fixed) a.fixed_o[q] = a.fixed[w];
+    const double D0 = a.D0[w];
+    const uint32_t tg = a.tag[w];
+    const int8_t fx = a.fixed ? a.fixed[w] : (int8_t)0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        a.x_o[(size_t)q * ND + d] = xv[d];
+        if (anchor_o) anchor_o[(size_t)q * ND + d] = xv[d];
+        a.m_o[(size_t)q * ND + d] = mv[d];
+        a.v_o[(size_t)q * ND + d] = vv[d];
+    }
+    a.D0_o[q] = D0;
+    a.tag_o[q] = tg;
+    if (a.fixed) a.fixed_o[q] = fx;
 }
 
 template <int ND>

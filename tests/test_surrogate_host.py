@@ -1,0 +1,58 @@
+"""Host-side parts of the surrogate pipeline (no GPU): features, warm-start
+options and the default rung-1 deadline."""
+import math
+
+import numpy as np
+import pytest
+
+from rcpgenerator_amd import surrogate as S
+from rcpgenerator_amd.census import distribution_stats
+
+
+def test_default_deadline_is_the_quick_preset():
+    # rcp_generator.cpp:5870-5885: 2*4000 + 9*sqrt(N*Ndim) - 2000, floored at 1500
+    assert S.default_deadline(10000, 3) == pytest.approx(6000 + 9 * math.sqrt(30000))
+    assert S.default_deadline(1, 1) == 6009.0
+    assert S.default_deadline(10**9, 8) > 6000
+
+
+def test_features_of_a_mono_periodic_packing():
+    f = S.features_of_packing(np.ones(1000), 3, [0, 0, 0])
+    assert f.dtype == np.float32 and f.shape == (S.N_FEATURES,)
+    assert f[0] == f[1] == f[2] == 0.0 and f[3] == 0.0        # no spread, ratio 1
+    assert list(f[4:8]) == [0.0, 1.0, 0.0, 0.0]               # one-hot Ndim = 3
+    assert f[8] == 0.0 and f[9] == 0.0                        # periodic, not curved
+    assert f[10] == pytest.approx(3.0 / 5.0) and f[11] == pytest.approx(3.0 / 5.0)
+
+
+def test_features_of_walls_curved_and_spread():
+    D = np.array([1.0, 3.0] * 50)
+    st = distribution_stats(D)
+    f = S.features(st, 2, [1, 0], 100)
+    assert f[8] == pytest.approx(0.5) and f[9] == 0.0
+    assert f[3] == pytest.approx(math.log(3.0))
+    assert f[0] == pytest.approx(st["m2"] - 1.0)
+    g = S.features(st, 3, [-3, -1, -1], 100)
+    assert g[8] == 0.0 and g[9] == 1.0
+
+
+def test_warm_options():
+    o = S.warm_options(0.64, 10000, 3)
+    assert o["initial_phi_target"] == pytest.approx(0.64 - S.WARM_DELTA)
+    assert o["rung1_deadline"] == int(S.SHORT_FRAC * S.default_deadline(10000, 3))
+    assert S.warm_options(0.5, 1, 1, frac=0.0)["rung1_deadline"] == 101   # above the hook's floor
+
+
+def test_committed_training_rows():
+    import os
+    d = np.load(os.path.join(os.path.dirname(S.__file__), "data", "surrogate_train.npz"),
+                allow_pickle=False)
+    X, Y, meta = d["X"], d["Y"], d["meta"]
+    assert X.shape == (len(Y), S.N_FEATURES) and Y.shape[1] == S.N_TARGETS
+    nd = meta[:, 2]
+    # one-hot dimension columns agree with the recorded dimension
+    assert np.array_equal(np.argmax(X[:, 4:8], axis=1) + 2, nd)
+    # phi_J within the known ranges of random close packing per dimension
+    for k, (lo, hi) in {2: (0.80, 0.91), 3: (0.55, 0.75), 4: (0.30, 0.55)}.items():
+        assert lo < Y[nd == k, 0].min() and Y[nd == k, 0].max() < hi
+    assert set(np.unique(Y[:, 1])) <= {0.0, 1.0}
