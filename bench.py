@@ -260,6 +260,49 @@ def bench_batch(args) -> int:
     return 0
 
 
+def bench_config1(args) -> int:
+    """The reference's own CPU-runnable case run to jamming: 3-D, N = 2000,
+    periodic cube, bidisperse 1.0 / 1.4 at 50/50, initial phi 0.08, seed 123,
+    default schedule.  Wall time of the whole run on the GPU engine and on
+    the reference (same input, host cores)."""
+    import rcpgenerator_amd as rcp
+    cfg0 = {"phi": 0.08, "N": 2000, "Ndim": 3, "box": [1.0, 1.0, 1.0], "walls": [0, 0, 0],
+            "seed": 123, "dist": {"type": "bidisperse", "d1": 1.0, "d2": 1.4, "p": 0.5}}
+    init = rcp.initialize_particles(cfg0)
+    inp = {"positions": init["positions"], "diameters": init["diameters"]}
+    cfg = {"box": init["box"], "walls": init["walls"]}
+    rcp.run_packing_observed(inp, cfg, options={"max_steps": 50})   # device warm-up
+    t0 = time.perf_counter()
+    r = rcp.run_packing_observed(inp, cfg)["result"]
+    dt = time.perf_counter() - t0
+    log(f"[bench] config 1 on the GPU: {r['steps']} steps in {dt:.2f}s, phi {r['phi']:.5f}")
+    line = {"metric": "wall seconds to jam, config 1 (3-D, N=2000, bidisperse)",
+            "value": dt, "unit": "s", "n_gpus": 1, "steps": r["steps"],
+            "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+            "phi": r["phi"], "max_min_dist": r["max_min_dist"],
+            "config": {"workload": "reference CPU case: 3-D, N=2000, periodic cube, bidisperse "
+                                   "1.0/1.4 at 50/50, initial phi 0.08, seed 123, run to jamming"}}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        if O.have_ref():
+            native = has_avx512() and os.path.exists(
+                os.path.join(O.REF_DIR, "native", "_rcpgenerator" + O.EXT_SUFFIX))
+            mod = O.ref_module(native=native)
+            cores = host_cores()
+            mod.set_num_threads(cores)
+            t0 = time.perf_counter()
+            rr = mod.run_packing_observed(inp, cfg)["result"]
+            tc = time.perf_counter() - t0
+            log(f"[bench] config 1 on the reference: {rr['steps']} steps in {tc:.2f}s")
+            line["cpu_baseline"] = {"value": tc, "unit": "s", "cores": cores, "kind": "reference",
+                                    "steps": rr["steps"], "phi": rr["phi"],
+                                    "sample": f"the whole run, {'AVX-512' if native else 'portable'}"
+                                              f" build, set_num_threads({cores})"}
+            line["cpu_over_gpu_time"] = round(tc / dt, 2)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def _ref_steps_time(payload):
     """Worker: seconds per step of the reference engine (one thread) on one
     packing, from runs of max_steps 1 and 1 + K."""
@@ -322,8 +365,9 @@ def main() -> int:
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     ap.add_argument("--workload", choices=("single", "batch"), default="single")
-    ap.add_argument("--config", choices=("2", "3", "3s"), default="2",
-                    help="single-packing configuration (2: the headline 2-D workload)")
+    ap.add_argument("--config", choices=("1", "2", "3", "3s"), default="2",
+                    help="single-packing configuration (2: the headline 2-D workload; 1: the "
+                         "reference's CPU case, run to jamming)")
     ap.add_argument("--packings", type=int, default=64)
     ap.add_argument("--threads", type=int, default=8)
     ap.add_argument("--pool", action="store_true",
@@ -342,6 +386,8 @@ def main() -> int:
 
     if args.workload == "batch":
         return bench_batch(args)
+    if args.config == "1":
+        return bench_config1(args)
     if args.copy_calibration:
         copy_calibration()
     import rcpgenerator_amd as rcp

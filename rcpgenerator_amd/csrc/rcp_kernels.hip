@@ -85,7 +85,9 @@ __device__ __forceinline__ double adam_dof(double Fk, double& m, double& v, doub
 
 __device__ __forceinline__ int cell_coord(double x, double L, double scale, int nc)
 {
-    double xw = x - L * floor(x / L);
+    // positions are wrapped by every step, so the division is rarely needed
+    // (x in [0, L) gives floor(x / L) = 0 and the same xw)
+    const double xw = (x >= 0.0 && x < L) ? x : x - L * floor(x / L);
     int k = (int)floor(xw * scale);
     if (k < 0) k = 0;
     if (k >= nc) k = nc - 1;
