@@ -1511,10 +1511,8 @@ __global__ __launch_bounds__(kWave) void k_exact_reduce(StepArgs a, const uint32
 // ---------------------------------------------------------------- binning
 
 template <int ND>
-__global__ __launch_bounds__(kBlock) void k_bin(BinArgs a)
+__device__ __forceinline__ void bin_one(const BinArgs& a, int w)
 {
-    const int w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= a.g.N) return;
     long long c = 0, stride = 1;
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
@@ -1525,6 +1523,13 @@ __global__ __launch_bounds__(kBlock) void k_bin(BinArgs a)
     }
     a.cell_of[w] = (uint32_t)c;
     a.slot[w] = atomicAdd(&a.cell_count[c], 1u);
+}
+
+template <int ND>
+__global__ __launch_bounds__(kBlock) void k_bin(BinArgs a)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w < a.g.N) bin_one<ND>(a, w);
 }
 
 template <int ND>
@@ -1727,6 +1732,39 @@ __global__ __launch_bounds__(kBlock) void k_permute(BinArgs a)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q < a.g.N) permute_one<ND>(a, q, nullptr);
+}
+
+// The whole re-binning of a small packing in one workgroup (the passes of
+// launch_rebin / launch_rebin_multi with workgroup barriers between them, so
+// one launch replaces nine).  Same cell order: counts, exclusive scan,
+// placement, per-cell order by previous position, gather.
+constexpr int kSmallRebinBlock = 1024;
+constexpr int kSmallRebinMaxN = 16384;
+constexpr long long kSmallRebinMaxCells = 65536;
+
+template <int ND, bool MULTI>
+__global__ __launch_bounds__(kSmallRebinBlock) void k_rebin_small(BinArgs a, MultiGrid mg)
+{
+    const int N = a.g.N;
+    const int T = (int)a.grid.total;
+    const int tid = threadIdx.x;
+    for (int c = tid; c <= T; c += kSmallRebinBlock) a.cell_count[c] = 0u;
+    __syncthreads();
+    for (int w = tid; w < N; w += kSmallRebinBlock) {
+        if (MULTI) bin_multi_one<ND>(a, mg, w);
+        else bin_one<ND>(a, w);
+    }
+    __syncthreads();
+    for (int c = tid; c < T; c += kSmallRebinBlock) a.cell_start[c] = a.cell_count[c];
+    __syncthreads();
+    block_scan_inplace<kSmallRebinBlock>(a.cell_start, T);
+    for (int w = tid; w < N; w += kSmallRebinBlock) place_one(a, w);
+    __syncthreads();
+    BinArgs b = a;
+    b.big = nullptr;   // crowded cells sorted in place (few particles here)
+    for (int c = tid; c < T; c += kSmallRebinBlock) cell_order_one(b, c, 0u);
+    __syncthreads();
+    for (int q = tid; q < N; q += kSmallRebinBlock) permute_one<ND>(a, q, nullptr);
 }
 
 // ------------------------------------------------------------------ lists
@@ -2330,6 +2368,19 @@ inline int check_launch()
         default: return 2;                                                             \
     }
 
+#define RCP_DISPATCH_ND_BOOL(nd, KERNEL, B, ...)                                       \
+    switch (nd) {                                                                      \
+        case 1: KERNEL<1, B> __VA_ARGS__; break;                                       \
+        case 2: KERNEL<2, B> __VA_ARGS__; break;                                       \
+        case 3: KERNEL<3, B> __VA_ARGS__; break;                                       \
+        case 4: KERNEL<4, B> __VA_ARGS__; break;                                       \
+        case 5: KERNEL<5, B> __VA_ARGS__; break;                                       \
+        case 6: KERNEL<6, B> __VA_ARGS__; break;                                       \
+        case 7: KERNEL<7, B> __VA_ARGS__; break;                                       \
+        case 8: KERNEL<8, B> __VA_ARGS__; break;                                       \
+        default: return false;                                                         \
+    }
+
 size_t step_partial_blocks(size_t n)
 {
     return (n + kDirectBlock - 1) / kDirectBlock;
@@ -2487,10 +2538,25 @@ static void cell_order(const BinArgs& a, hipStream_t s)
     if (a.big) k_big_cells<<<kBigGrid, kBigBlock, 0, s>>>(a);
 }
 
+static bool rebin_small(BinArgs& a, const MultiGrid* mg, hipStream_t s)
+{
+    if (a.g.N > kSmallRebinMaxN || a.grid.total > kSmallRebinMaxCells) return false;
+    const MultiGrid none;
+    if (mg) {
+        RCP_DISPATCH_ND_BOOL(a.g.ndim, k_rebin_small, true,
+                             <<<1, kSmallRebinBlock, 0, s>>>(a, *mg));
+    } else {
+        RCP_DISPATCH_ND_BOOL(a.g.ndim, k_rebin_small, false,
+                             <<<1, kSmallRebinBlock, 0, s>>>(a, none));
+    }
+    return true;
+}
+
 int launch_rebin(BinArgs& a, void* scan_tmp, hipStream_t s)
 {
     const int N = a.g.N;
     const size_t total = (size_t)a.grid.total;
+    if (rebin_small(a, nullptr, s)) return check_launch();
     k_fill_u32<<<grid_for(total + 1), kBlock, 0, s>>>(a.cell_count, 0u, total + 1);
     RCP_DISPATCH_ND(a.g.ndim, k_bin, <<<grid_for(N), kBlock, 0, s>>>(a));
     if (launch_scan_u32(a.cell_count, a.cell_start, total, scan_tmp, s)) return 1;
@@ -2504,6 +2570,7 @@ int launch_rebin_multi(BinArgs& a, const MultiGrid& mg, void* scan_tmp, hipStrea
 {
     const int N = a.g.N;
     const size_t total = (size_t)a.grid.total;
+    if (rebin_small(a, &mg, s)) return check_launch();
     k_fill_u32<<<grid_for(total + 1), kBlock, 0, s>>>(a.cell_count, 0u, total + 1);
     RCP_DISPATCH_ND(a.g.ndim, k_bin_multi, <<<grid_for(N), kBlock, 0, s>>>(a, mg));
     if (launch_scan_u32(a.cell_count, a.cell_start, total, scan_tmp, s)) return 1;
