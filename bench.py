@@ -299,6 +299,13 @@ def bench_config1(args) -> int:
                                     "sample": f"the whole run, {'AVX-512' if native else 'portable'}"
                                               f" build, set_num_threads({cores})"}
             line["cpu_over_gpu_time"] = round(tc / dt, 2)
+            mod.set_num_threads(1)
+            t0 = time.perf_counter()
+            r1 = mod.run_packing_observed(inp, cfg)["result"]
+            t1 = time.perf_counter() - t0
+            log(f"[bench] config 1 on the reference, 1 thread: {r1['steps']} steps in {t1:.2f}s")
+            line["cpu_baseline_1thread"] = {"value": t1, "unit": "s", "cores": 1,
+                                            "steps": r1["steps"], "phi": r1["phi"]}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -667,6 +667,22 @@ void Engine::recompute_d0_cache(const std::vector<double>& D0, const std::vector
     }
     const double advantage = (nd_ == 2 || nd_ == 3) ? 4.0 : 3.0;
     polydisperse_ = classes_ > 1 && single > advantage * multi;
+    // Tiles pay when most of them are interior (no periodic wrap in their
+    // halo, reach-limited stencils): at least 12 cells along the tile rows
+    // and 6 across, judged at the sizes of a packing grown to about phi_J.
+    // Smaller boxes (a few hundred to a few thousand particles) take the
+    // neighbour rows instead.
+    {
+        const double phi_j = nd_ == 2 ? 0.85 : 0.65;
+        const double g = (phi_ > 0.0 && phi_ < phi_j)
+            ? std::pow(phi_j / phi_, 1.0 / static_cast<double>(nd_)) : 1.0;
+        const double edge = (d0max + shell0_) * g * (1.0 + 1e-6);
+        tile_grid_ok_ = edge > 0.0 && !box_.empty();
+        for (std::size_t k = 0; k < nd_ && k < box_.size(); ++k) {
+            const double n = std::floor(box_[k] / edge);
+            if (n < (k == 0 ? 12.0 : 6.0)) tile_grid_ok_ = false;
+        }
+    }
     // particles that can land in the wide classes (D > dmax 2^-wide at any
     // binning; the margin absorbs the rounding of D0 kappa)
     const int wide = std::max(0, classes_ - 2);
@@ -747,7 +763,8 @@ bool Engine::multi_supported() const
 bool Engine::tiles_supported() const
 {
     if (opt_.exact_reductions || (nd_ != 2 && nd_ != 3)) return false;
-    return opt_.neighbor_mode == 3 || (opt_.neighbor_mode == 0 && !polydisperse_);
+    return opt_.neighbor_mode == 3 ||
+           (opt_.neighbor_mode == 0 && !polydisperse_ && tile_grid_ok_);
 }
 
 void Engine::plan_multi_grid(double skin, MultiGrid& mg) const

@@ -297,7 +297,7 @@ private:
     TilePlan tile_plan_;
     MultiGrid mg_;
     int classes_ = 1;
-    bool polydisperse_ = false;
+    bool polydisperse_ = false, tile_grid_ok_ = true;
     std::vector<long long> class_count_;
     uint64_t list_builds_ = 0, list_entries_ = 0, reorders_ = 0, direct_steps_total_ = 0,
              overflow_steps_ = 0;
