@@ -260,6 +260,28 @@ def bench_batch(args) -> int:
     return 0
 
 
+def bench_surrogate(args) -> int:
+    """Batched runs guided by the committed surrogate against plain batched
+    runs of the same held-out packings (seeds 900000.., N 500..2000, 2-4
+    dimensions): jammed packings per hour of wall time for both."""
+    from rcpgenerator_amd import surrogate as S
+    from rcpgenerator_amd.workloads import batch_cases
+    model = os.path.join(ROOT, "rcpgenerator_amd", "data", "surrogate.npz")
+    cases = batch_cases(args.packings, 900000, 500, 2000, (2, 3, 4))
+    out = S.evaluate(S.Surrogate.load(model), cases, log=log)
+    P = out["packings"]
+    value = P / (out["wall_guided_s"] / 3600.0)
+    line = {"metric": "jammed packings/hour (batched engine + surrogate warm start)",
+            "value": value, "unit": "packings/hour", "n_gpus": 1, "higher_is_better": True,
+            "plain_value": P / (out["wall_plain_s"] / 3600.0),
+            "vs_plain_wall": out["speedup_wall"], "vs_plain_steps": out["speedup_steps"],
+            "dtype": "f64", "data": "synthetic", "evaluation": out,
+            "config": {"workload": f"{P} held-out packings of the batched generator, N 500..2000, "
+                                   "2-4 dimensions; guided vs plain batched runs"}}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def bench_config1(args) -> int:
     """The reference's own CPU-runnable case run to jamming: 3-D, N = 2000,
     periodic cube, bidisperse 1.0 / 1.4 at 50/50, initial phi 0.08, seed 123,
@@ -371,7 +393,7 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
-    ap.add_argument("--workload", choices=("single", "batch"), default="single")
+    ap.add_argument("--workload", choices=("single", "batch", "surrogate"), default="single")
     ap.add_argument("--config", choices=("1", "2", "3", "3s"), default="2",
                     help="single-packing configuration (2: the headline 2-D workload; 1: the "
                          "reference's CPU case, run to jamming)")
@@ -393,6 +415,8 @@ def main() -> int:
 
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "surrogate":
+        return bench_surrogate(args)
     if args.config == "1":
         return bench_config1(args)
     if args.copy_calibration:

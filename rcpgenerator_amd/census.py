@@ -40,7 +40,8 @@ def census_row(*, diameters, ndim: int, walls: Sequence[int], result: dict,
     """One census row for a finished packing.  `result` needs phi,
     max_min_dist and steps (a run_packing dict or a batch summary);
     `diameters` are the packing's input diameters (their moments are
-    scale-free)."""
+    scale-free).  A census row (max_overlap instead of max_min_dist) serves
+    as `result` too."""
     stats = distribution_stats(diameters)
     phi = float(result["phi"])
     return {
@@ -57,7 +58,8 @@ def census_row(*, diameters, ndim: int, walls: Sequence[int], result: dict,
         "size_ratio": stats["d_max"] / stats["d_min"],
         "phi": phi,
         "phi_corr": phi,
-        "max_overlap": float(result["max_min_dist"]),
+        "max_overlap": float(result["max_min_dist"] if "max_min_dist" in result
+                             else result["max_overlap"]),
         "steps": int(result["steps"]) if result.get("steps") is not None else None,
         "wall_s": float(wall_s) if wall_s is not None else None,
         **stats,

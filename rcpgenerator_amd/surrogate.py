@@ -3,19 +3,19 @@
 A small network (``capi.Mlp``: 3 hidden layers of 128, FP32, trained on the
 GPU by this library's own forward and backward kernels) maps the diameter
 distribution's moments, the size ratio, the dimension and the boundary to
-two numbers: the jamming fraction phi_J the plain method reaches, and a
-score for whether a warm start with a short schedule keeps that phi_J.  A
-warm start (``warm_options``) begins the run WARM_DELTA below the predicted
-phi_J (``initial_phi_target``) with rung 1 cut to SHORT_FRAC of its preset
-deadline (``rung1_deadline``, the reference's RCP_RUNG1_DEADLINE hook,
-rcp_generator.cpp:5864-5868).  The plain growth and minimisation still
-finish every packing; ``evaluate`` judges each warm-started packing against
-the prediction and runs it again with the default schedule when the
-prediction missed.
+the jamming fraction phi_J the plain method reaches and, for each of a few
+shortened schedules (``SCHEDULES``: a cut rung-1 deadline, a larger mu, and
+optionally a start just below the predicted phi_J), a score for whether
+that schedule keeps phi_J within TOL.  The schedules use the reference's
+own knobs: ``rung1_deadline`` (its RCP_RUNG1_DEADLINE hook,
+rcp_generator.cpp:5864-5868), ``mu`` and ``initial_phi_target``
+(PackingRunOptions).  The plain growth and minimisation still finish every
+packing; ``evaluate`` judges each shortened run against the prediction and
+runs it again with the default schedule when the prediction missed.
 
 Training rows are census rows (``rcpgenerator_amd.census``) with the
-measured phi_J and the label ``safe`` (``campaign``).  Command line:
-``python -m rcpgenerator_amd.surrogate campaign | train | evaluate``.
+measured phi_J and one safety label per schedule (``campaign``).  Command
+line: ``python -m rcpgenerator_amd.surrogate campaign | train | evaluate``.
 """
 from __future__ import annotations
 
@@ -28,7 +28,7 @@ from . import capi
 from .census import distribution_stats
 
 N_FEATURES = 12
-N_TARGETS = 2
+N_TARGETS = 4   # phi_J and one safety score per shortened schedule
 HIDDEN = 128
 LAYERS = 3
 
@@ -132,7 +132,7 @@ class Surrogate:
         return np.asarray(losses, dtype=np.float32)
 
     def predict(self, X) -> np.ndarray:
-        """Rows of (phi_J, safe score) for feature rows X."""
+        """Rows of (phi_J, safety score of each schedule) for feature rows X."""
         X = np.asarray(X, dtype=np.float32).reshape(-1, N_FEATURES)
         P = self.net.predict((X - self.x_mean) / self.x_std) * self.y_std + self.y_mean
         P[:, 0] += self._base(X)
@@ -159,16 +159,32 @@ class Surrogate:
 # rows (config-4 families, N 500..2000): the predictor without the network
 DIM_MEAN_PHI = {2: 0.8519, 3: 0.6292, 4: 0.4141}
 
-WARM_DELTA = 0.04    # a warm start begins this far below the predicted phi_J
-SHORT_FRAC = 0.2     # and shortens rung 1 to this fraction of the preset deadline
-TOL = 0.002          # phi agreement with the plain method a warm start must keep
+MU_DEFAULT = 2.5e-4  # the reference's default mu (RG:4781-4787)
+TOL = 0.002          # phi agreement with the plain method a guided run must keep
+RERUN_TOL = 0.01     # a shortened run this far from the prediction is run again
+
+# Shortened schedules, fastest first: (name, fraction of the preset rung-1
+# deadline, mu factor, warm-start offset below the predicted phi_J or None
+# for a cold start).  A larger mu compresses harder during the shorter
+# rung 1; a warm start begins near phi_J.
+SCHEDULES = (
+    ("warm-0.25-mu4", 0.25, 4.0, 0.04),
+    ("warm-0.35-mu4", 0.35, 4.0, 0.04),
+    ("cold-0.5-mu2", 0.5, 2.0, None),
+)
+N_SCHEDULES = len(SCHEDULES)
+assert N_TARGETS == 1 + N_SCHEDULES
 
 
-def warm_options(phi_pred: float, N: int, ndim: int, frac: float = SHORT_FRAC,
-                 delta: float = WARM_DELTA) -> dict:
-    """Run options of a warm start from a predicted phi_J."""
-    return {"initial_phi_target": float(phi_pred) - delta,
-            "rung1_deadline": max(101, int(frac * default_deadline(N, ndim)))}
+def schedule_options(k: int, phi_pred: float, N: int, ndim: int) -> dict:
+    """Run options of schedule k for a packing of predicted jamming fraction
+    phi_pred."""
+    _, frac, mu, delta = SCHEDULES[k]
+    o = {"rung1_deadline": max(101, int(frac * default_deadline(N, ndim))),
+         "mu": MU_DEFAULT * mu}
+    if delta is not None:
+        o["initial_phi_target"] = float(phi_pred) - delta
+    return o
 
 
 def _initialize(cases, threads: int):
@@ -192,81 +208,129 @@ def _batch(inits, options, threads: int = 8):
     return res, wall, st
 
 
-def campaign(cases, threads: int = 8, log=print) -> list:
-    """Training rows: each packing runs with the plain method (phi_J, steps)
-    and once warm-started from its own phi_J with the short schedule; the
-    label `safe` says whether that warm start kept phi within TOL."""
+def campaign(cases, threads: int = 8, log=print, plain=None) -> list:
+    """Training rows (census rows plus labels): each packing runs with the
+    plain method (phi_J, steps; or `plain`, a list of such results already
+    measured) and once with every shortened schedule, warm-started from its
+    own phi_J.  `phi_sched` / `steps_sched` record those runs and
+    `safe_sched[k]` whether schedule k kept phi within TOL."""
     from .census import census_row
     inits = _initialize(cases, threads)
-    plain, wall_p, _ = _batch(inits, [None] * len(inits), threads)
-    log(f"[surrogate] plain batch: {len(cases)} packings in {wall_p:.1f}s")
-    warm, wall_w, _ = _batch(inits, [warm_options(r["phi"], c["N"], c["Ndim"])
-                                     for r, c in zip(plain, cases)], threads)
-    log(f"[surrogate] warm batch: {wall_w:.1f}s")
+    if plain is None:
+        plain, wall_p, _ = _batch(inits, [None] * len(inits), threads)
+        log(f"[surrogate] plain batch: {len(cases)} packings in {wall_p:.1f}s")
+    sched = []
+    for k in range(N_SCHEDULES):
+        res, wall, _ = _batch(inits, [schedule_options(k, p["phi"], c["N"], c["Ndim"])
+                                      for p, c in zip(plain, cases)], threads)
+        log(f"[surrogate] schedule {SCHEDULES[k][0]}: {wall:.1f}s")
+        sched.append(res)
     rows = []
-    for c, (x, D, b, w), p, q in zip(cases, inits, plain, warm):
+    for i, (c, (x, D, b, w), p) in enumerate(zip(cases, inits, plain)):
         row = census_row(diameters=D, ndim=c["Ndim"], walls=list(w), result=p,
                          search_name="surrogate-campaign", model_name=c["dist"]["type"],
                          candidate=c["seed"], seed=c["seed"])
-        row.update({"phi_warm": q["phi"], "steps_warm": q["steps"],
-                    "safe": bool(abs(q["phi"] - p["phi"]) <= TOL)})
+        phis = [float(sched[k][i]["phi"]) for k in range(N_SCHEDULES)]
+        row.update({"phi_sched": phis,
+                    "steps_sched": [int(sched[k][i]["steps"]) for k in range(N_SCHEDULES)],
+                    "safe_sched": [bool(abs(q - p["phi"]) <= TOL) for q in phis]})
         rows.append(row)
     return rows
 
 
 def training_arrays(rows):
+    """Features and targets (phi_J, then one 0/1 safety label per schedule)."""
     X = np.stack([features_of_row(r) for r in rows])
-    Y = np.array([[r["phi"], 1.0 if r["safe"] else 0.0] for r in rows], dtype=np.float32)
+    Y = np.array([[r["phi"]] + [1.0 if v else 0.0 for v in r["safe_sched"]] for r in rows],
+                 dtype=np.float32)
     return X, Y
+
+
+def choose_schedule(pred_row) -> int:
+    """The fastest schedule the network calls safe (score >= 0.5), or -1
+    for the default schedule."""
+    for k in range(N_SCHEDULES):
+        if pred_row[1 + k] >= 0.5:
+            return k
+    return -1
 
 
 def evaluate(surr: "Surrogate", cases, threads: int = 8, log=print, detail: bool = False) -> dict:
     """Plain batched runs against surrogate-guided ones on the same packings.
-    Guided: packings the network calls safe start warm with the short
-    schedule, the others plain; a warm run whose phi misses the prediction
-    by more than 2 TOL is run again with the plain method."""
+    Guided: each packing takes the fastest schedule the network calls safe
+    (or the default one); a shortened run whose phi misses the predicted
+    phi_J by more than RERUN_TOL is run again with the default schedule, and
+    its cost counts both runs.  Also reported: every packing on the safest
+    shortened schedule, the method without the network."""
     inits = _initialize(cases, threads)
     plain, wall_p, _ = _batch(inits, [None] * len(inits), threads)
     X = np.stack([features_of_packing(D, c["Ndim"], list(w)) for c, (x, D, b, w) in
                   zip(cases, inits)])
     pred = surr.predict(X)
-    warm = pred[:, 1] >= 0.5
-    opts = [warm_options(p[0], c["N"], c["Ndim"]) if wm else None
-            for p, c, wm in zip(pred, cases, warm)]
+    choice = [choose_schedule(p) for p in pred]
+    opts = [schedule_options(k, p[0], c["N"], c["Ndim"]) if k >= 0 else None
+            for k, p, c in zip(choice, pred, cases)]
     guided, wall_g, _ = _batch(inits, opts, threads)
-    miss = [k for k in range(len(cases))
-            if warm[k] and abs(guided[k]["phi"] - pred[k, 0]) > 2 * TOL]
+    miss = [i for i in range(len(cases))
+            if choice[i] >= 0 and abs(guided[i]["phi"] - pred[i, 0]) > RERUN_TOL]
     wall_r = 0.0
     if miss:
-        again, wall_r, _ = _batch([inits[k] for k in miss], [None] * len(miss), threads)
-        for k, r in zip(miss, again):
-            guided[k] = dict(r, steps=r["steps"] + guided[k]["steps"])
+        again, wall_r, _ = _batch([inits[i] for i in miss], [None] * len(miss), threads)
+        for i, r in zip(miss, again):
+            guided[i] = dict(r, steps=r["steps"] + guided[i]["steps"])
     phi_p = np.array([r["phi"] for r in plain])
     phi_g = np.array([r["phi"] for r in guided])
+    # the method without the network: every packing on the safest shortened
+    # schedule (cold start, so no phi_J prediction is needed)
+    kf = N_SCHEDULES - 1
+    fixed, wall_f, _ = _batch(inits, [schedule_options(kf, 0.0, c["N"], c["Ndim"])
+                                      for c in cases], threads)
+    phi_f = np.array([r["phi"] for r in fixed])
+    steps_f = sum(r["steps"] for r in fixed)
     # the predictor without the network: the mean training phi_J of the dimension
     base = np.array([DIM_MEAN_PHI.get(int(c["Ndim"]), float(phi_p.mean())) for c in cases])
     steps_p = sum(r["steps"] for r in plain)
     steps_g = sum(r["steps"] for r in guided)
     err = np.abs(pred[:, 0] - phi_p)
     out = {
-        "packings": len(cases), "warm": int(warm.sum()), "reruns": len(miss),
+        "packings": len(cases),
+        "schedules": {SCHEDULES[k][0]: int(sum(1 for c in choice if c == k))
+                      for k in range(N_SCHEDULES)},
+        "default": int(sum(1 for c in choice if c < 0)), "reruns": len(miss),
         "wall_plain_s": wall_p, "wall_guided_s": wall_g + wall_r,
         "steps_plain": int(steps_p), "steps_guided": int(steps_g),
         "speedup_steps": steps_p / steps_g, "speedup_wall": wall_p / (wall_g + wall_r),
         "phi_within_tol": float(np.mean(np.abs(phi_g - phi_p) <= TOL)),
+        "phi_mean_diff": float(np.mean(phi_g - phi_p)),
         "phi_max_abs_diff": float(np.max(np.abs(phi_g - phi_p))),
         "pred_mae": float(err.mean()), "pred_max_err": float(err.max()),
         "dim_mean_mae": float(np.abs(base - phi_p).mean()),
         "jammed_guided": float(np.mean([r["max_min_dist"] < 5e-4 for r in guided])),
+        "fixed_schedule": {"name": SCHEDULES[kf][0], "wall_s": wall_f,
+                           "speedup_steps": steps_p / steps_f, "speedup_wall": wall_p / wall_f,
+                           "phi_within_tol": float(np.mean(np.abs(phi_f - phi_p) <= TOL)),
+                           "phi_mean_diff": float(np.mean(phi_f - phi_p)),
+                           "jammed": float(np.mean([r["max_min_dist"] < 5e-4 for r in fixed]))},
     }
     log(f"[surrogate] evaluation: {out}")
     if detail:
-        out.update({"warm_ids": [int(k) for k in np.nonzero(warm)[0]], "rerun_ids": miss,
+        out.update({"choice": choice, "rerun_ids": miss,
                     "steps_plain_each": [r["steps"] for r in plain],
                     "steps_guided_each": [r["steps"] for r in guided],
                     "phi_plain_each": phi_p.tolist(), "phi_guided_each": phi_g.tolist(),
                     "pred_each": pred.tolist()})
     return out
+
+
+def save_rows_npz(path, rows) -> None:
+    """Compact training file: features, targets and the campaign's record."""
+    X, Y = training_arrays(rows)
+    meta = np.array([[r["seed"], r["N"], r["Ndim"], r["steps"]] for r in rows], dtype=np.int64)
+    np.savez_compressed(path, X=X, Y=Y, meta=meta,
+                        phi_sched=np.array([r["phi_sched"] for r in rows]),
+                        steps_sched=np.array([r["steps_sched"] for r in rows], dtype=np.int64),
+                        moments=np.array([[r[f"m{k}"] for k in range(1, 7)] +
+                                          [r["d_min"], r["d_max"]] for r in rows]))
 
 
 def _main(argv=None) -> int:
@@ -281,10 +345,13 @@ def _main(argv=None) -> int:
     c.add_argument("--seed0", type=int, default=200000)
     c.add_argument("--nmin", type=int, default=500)
     c.add_argument("--nmax", type=int, default=2000)
-    c.add_argument("--dims", default="2,3,4,5")
-    c.add_argument("--chunk", type=int, default=192, help="packings per batch")
-    c.add_argument("--out", required=True)
-    t = sub.add_parser("train", help="train the network on census rows")
+    c.add_argument("--dims", default="2,3,4")
+    c.add_argument("--chunk", type=int, default=160, help="packings per batch")
+    c.add_argument("--plain", help="census JSONL with the plain runs of these packings "
+                                   "(matched by seed; they are not run again)")
+    c.add_argument("--out", required=True, help="census JSONL to append to")
+    c.add_argument("--npz", help="also write the compact training file")
+    t = sub.add_parser("train", help="train the network on census rows or a training file")
     t.add_argument("--data", required=True, nargs="+")
     t.add_argument("--out", required=True)
     t.add_argument("--epochs", type=int, default=300)
@@ -294,19 +361,31 @@ def _main(argv=None) -> int:
     e.add_argument("--seed0", type=int, default=900000)
     e.add_argument("--nmin", type=int, default=500)
     e.add_argument("--nmax", type=int, default=2000)
-    e.add_argument("--dims", default="2,3,4,5")
+    e.add_argument("--dims", default="2,3,4")
     e.add_argument("--out")
     a = ap.parse_args(argv)
     if a.cmd == "campaign":
         dims = tuple(int(v) for v in a.dims.split(","))
         cases = batch_cases(a.packings, a.seed0, a.nmin, a.nmax, dims)
-        total = safe = 0
+        known = {}
+        if a.plain:
+            known = {int(r["seed"]): r for r in census.load_rows(a.plain)}
+        all_rows = []
         for k in range(0, len(cases), a.chunk):
-            rows = campaign(cases[k:k + a.chunk], log=lambda m: print(m, flush=True))
+            part = cases[k:k + a.chunk]
+            plain = None
+            if known:
+                plain = [known[c["seed"]] for c in part]
+                for c, r in zip(part, plain):
+                    if int(r["N"]) != c["N"] or int(r["Ndim"]) != c["Ndim"]:
+                        raise ValueError(f"plain row of seed {c['seed']} is another packing")
+            rows = campaign(part, log=lambda m: print(m, flush=True), plain=plain)
             census.append_rows(a.out, rows)
-            total += len(rows)
-            safe += sum(r["safe"] for r in rows)
-            print(json.dumps({"rows": total, "safe": safe}), flush=True)
+            all_rows.extend(rows)
+            safe = np.sum([r["safe_sched"] for r in all_rows], axis=0).tolist()
+            print(json.dumps({"rows": len(all_rows), "safe_per_schedule": safe}), flush=True)
+        if a.npz:
+            save_rows_npz(a.npz, all_rows)
     elif a.cmd == "train":
         Xs, Ys = [], []
         for p in a.data:
@@ -319,16 +398,16 @@ def _main(argv=None) -> int:
                 Xs.append(X1)
                 Ys.append(Y1)
         X, Y = np.concatenate(Xs), np.concatenate(Ys)
-        rows = X
         s = Surrogate(seed=1)
         losses = s.fit(X, Y, epochs=a.epochs)
         s.save(a.out)
         P = s.predict(X)
-        print(json.dumps({"rows": len(rows), "loss_first": float(losses[0]),
-                          "loss_last": float(losses[-1]), "val_loss": s.val_loss,
+        acc = [float(np.mean((P[:, 1 + k] >= 0.5) == (Y[:, 1 + k] >= 0.5)))
+               for k in range(N_SCHEDULES)]
+        print(json.dumps({"rows": len(X), "loss_first": float(losses[0]),
+                          "loss_last": float(losses[-1]),
                           "train_phi_mae": float(np.mean(np.abs(P[:, 0] - Y[:, 0]))),
-                          "train_safe_acc": float(np.mean((P[:, 1] >= 0.5) == (Y[:, 1] >= 0.5)))}),
-              flush=True)
+                          "train_safe_acc": acc}), flush=True)
     else:
         dims = tuple(int(v) for v in a.dims.split(","))
         out = evaluate(Surrogate.load(a.model), batch_cases(a.packings, a.seed0, a.nmin, a.nmax,

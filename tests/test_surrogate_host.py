@@ -36,11 +36,24 @@ def test_features_of_walls_curved_and_spread():
     assert g[8] == 0.0 and g[9] == 1.0
 
 
-def test_warm_options():
-    o = S.warm_options(0.64, 10000, 3)
-    assert o["initial_phi_target"] == pytest.approx(0.64 - S.WARM_DELTA)
-    assert o["rung1_deadline"] == int(S.SHORT_FRAC * S.default_deadline(10000, 3))
-    assert S.warm_options(0.5, 1, 1, frac=0.0)["rung1_deadline"] == 101   # above the hook's floor
+def test_schedule_options():
+    for k, (name, frac, mu, delta) in enumerate(S.SCHEDULES):
+        o = S.schedule_options(k, 0.64, 10000, 3)
+        assert o["rung1_deadline"] == int(frac * S.default_deadline(10000, 3))
+        assert o["mu"] == pytest.approx(S.MU_DEFAULT * mu)
+        if delta is None:
+            assert "initial_phi_target" not in o
+        else:
+            assert o["initial_phi_target"] == pytest.approx(0.64 - delta)
+    # fastest first: the deadline fractions do not decrease
+    fr = [f for _, f, _, _ in S.SCHEDULES]
+    assert fr == sorted(fr)
+
+
+def test_choose_schedule():
+    assert S.choose_schedule([0.6, 0.9, 0.9, 0.9]) == 0
+    assert S.choose_schedule([0.6, 0.1, 0.7, 0.9]) == 1
+    assert S.choose_schedule([0.6, 0.1, 0.2, 0.3]) == -1
 
 
 def test_committed_training_rows():
@@ -55,4 +68,7 @@ def test_committed_training_rows():
     # phi_J within the known ranges of random close packing per dimension
     for k, (lo, hi) in {2: (0.80, 0.91), 3: (0.55, 0.75), 4: (0.30, 0.55)}.items():
         assert lo < Y[nd == k, 0].min() and Y[nd == k, 0].max() < hi
-    assert set(np.unique(Y[:, 1])) <= {0.0, 1.0}
+    assert set(np.unique(Y[:, 1:])) <= {0.0, 1.0}
+    # the labels restate the recorded schedule runs
+    safe = np.abs(d["phi_sched"] - Y[:, :1]) <= S.TOL
+    assert np.array_equal(safe.astype(np.float32), Y[:, 1:])
