@@ -19,8 +19,8 @@ namespace rcp {
 
 constexpr int kMaxDim = 8;
 
-// Scalar slots of one fused step (block partials folded by the last block
-// in block order: deterministic run to run).
+// Scalar slots of one fused step (each workgroup stores a row of partials;
+// k_fold combines the rows in a fixed order: deterministic run to run).
 enum StepScalar : int {
     S_U = 0,      // pair overlaps (tag_j > tag_i) + wall / curved terms
     S_LC,         // sum of r_ij over pair contacts
@@ -33,7 +33,7 @@ enum StepScalar : int {
     S_MAXOV,      // max pair overlap
     S_MAXDISP2,   // max |x_new - x_anchor|^2 (minimum image)
     S_MAXSTEP2,   // max |x_new - x_old|^2 (minimum image)
-    S_OVERFLOW,   // particles whose contact buffer overflowed (direct mode)
+    S_OVERFLOW,   // particles whose contact buffer overflowed (selection path taken)
     S_NUM
 };
 
