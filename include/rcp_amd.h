@@ -76,6 +76,8 @@ typedef struct rcp_run_config {
     double osc_period_factor;     /* RCP_OSC_PERIOD_FACTOR (rcp_generator.cpp:5289-5294):
                                      period = factor * rung-1 end step, 0 < f < 100;
                                      0 = preset (0.5) */
+    size_t tile_particles;        /* tiled step: target particles per workgroup tile
+                                     (0 = the autotuned table; for the autotuner) */
 } rcp_run_config;
 
 /* Fills the reference defaults (reorder_interval 2000, everything else 0). */

@@ -57,6 +57,7 @@ class RunConfig(C.Structure):
         ("rung1_deadline", C.c_size_t),
         ("osc_cycles", C.c_size_t),
         ("osc_period_factor", C.c_double),
+        ("tile_particles", C.c_size_t),
     ]
 
 
@@ -232,7 +233,7 @@ class Engine:
     def __init__(self, positions, diameters, box=None, walls=None, fixed=None, fix_height=False,
                  max_steps=0, mu=None, fix_diameter=False, target_phi=None,
                  reorder_interval=2000, initial_phi_target=None, exact=False, neighbor_max=0,
-                 time_kernels=False, neighbor_mode=0):
+                 time_kernels=False, neighbor_mode=0, tile_particles=0):
         self._x = _f64(positions)
         self.N, self.ndim = self._x.shape
         self._D = _f64(diameters)
@@ -260,6 +261,7 @@ class Engine:
         c.exact_reductions = int(exact)
         c.time_kernels = int(time_kernels)
         c.neighbor_mode = int(neighbor_mode)
+        c.tile_particles = int(tile_particles)
         self._cfg = c
         h = C.c_void_p()
         _check(lib().rcp_engine_create(C.byref(c), C.byref(h)))

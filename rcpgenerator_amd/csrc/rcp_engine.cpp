@@ -903,7 +903,8 @@ void Engine::tile_rebin(double skin)
     // the skin (diameter growth and box stretch included).
     begin_timer(0);
     st_->rebin(geo_, dmax_now() + skin);
-    if (!plan_tiles((int)nd_, st_->grid, (long long)N_, has_fixed_ ? 1 : 0, tile_plan_))
+    if (!plan_tiles((int)nd_, st_->grid, (long long)N_, has_fixed_ ? 1 : 0,
+                    (int)opt_.tile_particles, tile_plan_))
         throw RtError("rcp: no tile plan for this grid");
     if (skin > 0.0)
         hip_check(hipMemcpyAsync(st_->anchor, st_->x, N_ * nd_ * sizeof(double),

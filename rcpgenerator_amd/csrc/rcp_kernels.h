@@ -116,8 +116,10 @@ struct TilePlan {
     long long tiles = 0, blocks = 0;   // tiles; persistent workgroups
     size_t lds_bytes = 0;
 };
-// false when the tiled step does not handle this shape (ndim not 2 or 3)
-bool plan_tiles(int ndim, const CellGrid& grid, long long N, int use_rk, TilePlan& p);
+// false when the tiled step does not handle this shape (ndim not 2 or 3).
+// per_tile: target particles per tile (0: the autotuned table)
+bool plan_tiles(int ndim, const CellGrid& grid, long long N, int use_rk, int per_tile,
+                TilePlan& p);
 int launch_step_tile(const StepArgs& a, const TilePlan& p, hipStream_t s,
                      hipEvent_t mark = nullptr);
 

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include "rcp_kernels.h"
+#include "rcp_tile_table.h"
 
 namespace rcp {
 
@@ -2415,9 +2416,11 @@ long long tile_grid(int ndim, size_t lds_bytes)
     return c_val;
 }
 
-bool plan_tiles(int ndim, const CellGrid& grid, long long N, int use_rk, TilePlan& p)
+bool plan_tiles(int ndim, const CellGrid& grid, long long N, int use_rk, int per_tile,
+                TilePlan& p)
 {
     if (ndim != 2 && ndim != 3) return false;
+    if (per_tile <= 0) per_tile = tuned_tile_particles(ndim);
     p.block = kTileBlock;
     const int nx = grid.nc[0];
     const int span_x = nx >= 3 ? 1 : 0;
@@ -2426,7 +2429,7 @@ bool plan_tiles(int ndim, const CellGrid& grid, long long N, int use_rk, TilePla
     const double rho = (double)N / (double)grid.total;   // particles per cell
     const int hc_max = 1024;
     const int t_max = hc_max / nrows - 2 * span_x;
-    int T = (int)(kTileBlock / (rho > 1e-9 ? rho : 1e-9));
+    int T = (int)(per_tile / (rho > 1e-9 ? rho : 1e-9));
     if (T < 1) T = 1;
     if (T > t_max) T = t_max;
     if (T > nx) T = nx;

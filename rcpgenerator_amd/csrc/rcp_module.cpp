@@ -139,6 +139,9 @@ void parse_run(const py::dict& input, const py::dict& config, const py::dict& op
     take(options, "osc_period_factor", osc_period_factor);
     c.osc_cycles = osc_cycles;
     c.osc_period_factor = osc_period_factor;
+    size_t tile_particles = 0;
+    take(options, "tile_particles", tile_particles);
+    c.tile_particles = tile_particles;
     take(options, "fix_diameter", fix_diameter);
     take(options, "reorder_interval", reorder_interval);
     c.max_steps = max_steps;
