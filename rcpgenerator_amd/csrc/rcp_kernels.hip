@@ -1338,8 +1338,14 @@ __device__ __forceinline__ void store_row(const StepArgs& a, unsigned blk, doubl
     }
 }
 
+// Three waves per SIMD fit the LDS of k_step_wide, but unconstrained the
+// kernel takes about 190 registers (two waves).  The occupancy target holds
+// the register count to three waves where that costs no spills: d <= 3 for
+// the single-packing kernel, d <= 2 for the batched one (its table lookup
+// needs more registers).
 template <int ND>
-__global__ __launch_bounds__(kWideBlock) void k_step_wide(StepArgs a, MultiGrid mg)
+__global__ __launch_bounds__(kWideBlock) __attribute__((amdgpu_waves_per_eu(ND <= 3 ? 3 : 1)))
+void k_step_wide(StepArgs a, MultiGrid mg)
 {
     __shared__ WideLds<ND> L;
     double acc[S_NUM];
@@ -2249,8 +2255,8 @@ __global__ __launch_bounds__(kMultiBlock) void kb_step_multi(const BatchPack* __
 }
 
 template <int ND>
-__global__ __launch_bounds__(kWideBlock) void kb_step_wide(const BatchPack* __restrict__ packs,
-                                                          BatchTable t)
+__global__ __launch_bounds__(kWideBlock) __attribute__((amdgpu_waves_per_eu(ND <= 2 ? 3 : 1)))
+void kb_step_wide(const BatchPack* __restrict__ packs, BatchTable t)
 {
     __shared__ WideLds<ND> L;
     const int e = table_entry(t, blockIdx.x);
