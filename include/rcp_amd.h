@@ -78,6 +78,9 @@ typedef struct rcp_run_config {
                                      0 = preset (0.5) */
     size_t tile_particles;        /* tiled step: target particles per workgroup tile
                                      (0 = the autotuned table; for the autotuner) */
+    size_t tile_split;            /* tiled step: cells of 1/split of the contact reach
+                                     per axis, stencils of +-split cells (1..3;
+                                     0 = the autotuned table; for the autotuner) */
 } rcp_run_config;
 
 /* Fills the reference defaults (reorder_interval 2000, everything else 0). */

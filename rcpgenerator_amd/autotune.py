@@ -1,10 +1,14 @@
 """Tile autotuner for the tiled step kernel.
 
-Times ``k_step_tile`` (HIP events around every step launch) for a sweep of
-tile sizes -- the target number of particles per workgroup tile -- on one 2-D
-and one 3-D packing of 10^6 particles, steps 1..K, and writes the fastest
-size per dimension as the compiled-in table ``csrc/rcp_tile_table.h``
-(the timings go into the header's comment).  Run on the GPU:
+Times ``k_step_tile`` (HIP events around every step launch) and the
+re-binning for a sweep of tile shapes -- the cell split (cells of 1/split of
+the contact reach, stencils of +-split cells) and the target number of
+particles per workgroup tile -- on one 2-D and one 3-D packing of 10^6
+particles, steps 1..K, and writes the fastest shape per dimension as the
+compiled-in table ``csrc/rcp_tile_table.h`` (the timings go into the
+header's comment).  A shape is priced by step kernel plus re-binning time
+per step: finer cells shorten the stencil scans but add cells to every
+re-binning.  Run on the GPU:
 
     python -m rcpgenerator_amd.autotune --out rcpgenerator_amd/csrc/rcp_tile_table.h
 
@@ -19,7 +23,8 @@ import math
 import time
 
 CANDIDATES = (64, 96, 128, 192, 256, 384)
-DEFAULT = 128    # kept unless another size is at least MARGIN faster
+SPLITS = (1, 2)
+DEFAULT = (1, 128)   # (split, particles per tile), kept unless another shape is MARGIN faster
 MARGIN = 0.03
 
 WORKLOADS = {
@@ -32,34 +37,43 @@ WORKLOADS = {
 
 
 def sweep(nd: int, steps: int, log=print) -> dict:
+    """{(split, per_tile): timings} for the workload of dimension nd."""
     from . import capi
     x, D, box, walls = capi.initialize(WORKLOADS[nd])
     out = {}
-    for t in CANDIDATES:
-        e = capi.Engine(x, D, box=box, walls=walls, max_steps=10 ** 6, time_kernels=True,
-                        tile_particles=t)
-        e.sync()
-        t0 = time.perf_counter()
-        e.run(steps)
-        e.sync()
-        wall = time.perf_counter() - t0
-        tm = e.timing()
-        st = e.stats()
-        out[t] = {"kernel_ms": tm["step_ms"] / max(tm["step_launches"], 1),
-                  "wall_ms_per_step": 1e3 * wall / steps,
-                  "rebin_ms_per_step": tm["build_ms"] / steps, "tile_steps": st["tile_steps"]}
-        log(f"[autotune] ndim {nd} tile {t}: kernel {out[t]['kernel_ms']:.4f} ms, "
-            f"step {out[t]['wall_ms_per_step']:.4f} ms")
-        del e
+    for sp in SPLITS:
+        for t in CANDIDATES:
+            e = capi.Engine(x, D, box=box, walls=walls, max_steps=10 ** 6, time_kernels=True,
+                            tile_particles=t, tile_split=sp)
+            e.sync()
+            t0 = time.perf_counter()
+            e.run(steps)
+            e.sync()
+            wall = time.perf_counter() - t0
+            tm = e.timing()
+            st = e.stats()
+            r = {"kernel_ms": tm["step_ms"] / max(tm["step_launches"], 1),
+                 "wall_ms_per_step": 1e3 * wall / steps,
+                 "rebin_ms_per_step": tm["build_ms"] / steps, "tile_steps": st["tile_steps"]}
+            r["cost_ms"] = r["kernel_ms"] + r["rebin_ms_per_step"]
+            out[(sp, t)] = r
+            log(f"[autotune] ndim {nd} split {sp} tile {t}: kernel {r['kernel_ms']:.4f} ms, "
+                f"re-binning {r['rebin_ms_per_step']:.4f} ms/step, "
+                f"step {r['wall_ms_per_step']:.4f} ms")
+            del e
     return out
 
 
-HEADER = """// Particles per workgroup tile of the tiled step, per dimension, as measured
-// by the tile autotuner (python -m rcpgenerator_amd.autotune on one MI355X:
-// 10^6 particles, steps 1..{steps}; average k_step_tile launch in ms):
-{timings}// A size replaces the default 128 only when it is at least 3 % faster.
-// The table is compiled in so that a run's tiling, and with it the rounding
-// of its fast-mode scalars, does not depend on timing noise at run time.
+HEADER = """// Tile shape of the tiled step per dimension -- cell split (cells of 1/split
+// of the contact reach, stencils of +-split cells) and particles per
+// workgroup tile -- as measured by the tile autotuner (python -m
+// rcpgenerator_amd.autotune on one MI355X: 10^6 particles, steps 1..{steps};
+// average k_step_tile launch plus re-binning per step, in ms, per
+// split/particles):
+{timings}// A shape replaces the default (split 1, 128 particles) only when it is at
+// least 3 % faster.  The table is compiled in so that a run's tiling, and with
+// it the rounding of its fast-mode scalars, does not depend on timing noise
+// at run time.
 #pragma once
 
 namespace rcp {{
@@ -67,6 +81,7 @@ namespace rcp {{
 struct TileTuning {{
     int ndim;
     int per_tile;   // target particles per tile (tile_len = per_tile / particles per cell)
+    int split;      // cells of 1/split of the contact reach
 }};
 
 constexpr TileTuning kTileTable[] = {{
@@ -79,17 +94,29 @@ inline int tuned_tile_particles(int ndim)
     return 128;
 }}
 
+inline int tuned_tile_split(int ndim)
+{{
+    for (const TileTuning& t : kTileTable)
+        if (t.ndim == ndim) return t.split;
+    return 1;
+}}
+
 }}  // namespace rcp
 """
 
 
-def choose(r: dict) -> int:
-    """The fastest tile size, unless it beats DEFAULT by less than MARGIN
-    (timing noise must not move the tiling)."""
-    best = min(r, key=lambda t: r[t]["kernel_ms"])
-    if DEFAULT in r and r[best]["kernel_ms"] > (1.0 - MARGIN) * r[DEFAULT]["kernel_ms"]:
+def choose(r: dict) -> tuple:
+    """The cheapest (split, per_tile) shape, unless it beats DEFAULT by less
+    than MARGIN (timing noise must not move the tiling)."""
+    best = min(r, key=lambda k: r[k]["cost_ms"])
+    if DEFAULT in r and r[best]["cost_ms"] > (1.0 - MARGIN) * r[DEFAULT]["cost_ms"]:
         return DEFAULT
     return best
+
+
+def _key(k: str) -> tuple:
+    sp, t = k.split("/")
+    return int(sp), int(t)
 
 
 def main(argv=None) -> int:
@@ -104,17 +131,21 @@ def main(argv=None) -> int:
     if a.results:
         with open(a.results) as f:
             last = [ln for ln in f if ln.startswith("{")][-1]
-        results = {int(nd): {int(t): v for t, v in r.items()}
+        results = {int(nd): {_key(k): v for k, v in r.items()}
                    for nd, r in json.loads(last)["results"].items()}
     for nd in (2, 3):
         r = results[nd] if a.results else sweep(nd, a.steps, log=lambda m: print(m, flush=True))
         results[nd] = r
         best[nd] = choose(r)
-        timings += f"//   {nd}-D: " + ", ".join(f"{t}: {r[t]['kernel_ms']:.4f}" for t in r) + "\n"
-    rows = "".join(f"    {{{nd}, {best[nd]}}},\n" for nd in (2, 3))
+        for sp in sorted({k[0] for k in r}):
+            timings += f"//   {nd}-D split {sp}: " + ", ".join(
+                f"{t}: {r[(sp, t)]['cost_ms']:.4f}" for (s2, t) in sorted(r) if s2 == sp) + "\n"
+    rows = "".join(f"    {{{nd}, {best[nd][1]}, {best[nd][0]}}},\n" for nd in (2, 3))
     with open(a.out, "w") as f:
         f.write(HEADER.format(steps=a.steps, timings=timings, rows=rows))
-    print(json.dumps({"best": best, "results": results}), flush=True)
+    print(json.dumps({"best": {nd: list(b) for nd, b in best.items()},
+                      "results": {nd: {f"{k[0]}/{k[1]}": v for k, v in r.items()}
+                                  for nd, r in results.items()}}), flush=True)
     return 0
 
 

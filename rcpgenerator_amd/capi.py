@@ -58,6 +58,7 @@ class RunConfig(C.Structure):
         ("osc_cycles", C.c_size_t),
         ("osc_period_factor", C.c_double),
         ("tile_particles", C.c_size_t),
+        ("tile_split", C.c_size_t),
     ]
 
 
@@ -233,7 +234,8 @@ class Engine:
     def __init__(self, positions, diameters, box=None, walls=None, fixed=None, fix_height=False,
                  max_steps=0, mu=None, fix_diameter=False, target_phi=None,
                  reorder_interval=2000, initial_phi_target=None, exact=False, neighbor_max=0,
-                 time_kernels=False, neighbor_mode=0, tile_particles=0):
+                 time_kernels=False, neighbor_mode=0, tile_particles=0,
+                 tile_split=0):
         self._x = _f64(positions)
         self.N, self.ndim = self._x.shape
         self._D = _f64(diameters)
@@ -262,6 +264,7 @@ class Engine:
         c.time_kernels = int(time_kernels)
         c.neighbor_mode = int(neighbor_mode)
         c.tile_particles = int(tile_particles)
+        c.tile_split = int(tile_split)
         self._cfg = c
         h = C.c_void_p()
         _check(lib().rcp_engine_create(C.byref(c), C.byref(h)))

@@ -9,6 +9,7 @@
 #include "rcp_amd.h"
 #include "rcp_engine.h"
 #include "rcp_kernels.h"
+#include "rcp_tile_table.h"
 
 #include <algorithm>
 #include <cmath>
@@ -214,9 +215,19 @@ void DeviceState::upload(const double* xh, const double* D0h, const int8_t* fixe
     sync();
 }
 
-void DeviceState::rebin(const Geometry& g, double reach)
+void DeviceState::rebin(const Geometry& g, double reach, int split)
 {
-    make_grid(g.box, nd, reach, std::max<long long>(64, 2 * (long long)N), grid);
+    const long long cap = std::max<long long>(64, 2 * (long long)N);
+    grid_span = 1;
+    if (split > 1) {
+        long long c = cap;
+        for (int d = 0; d < nd; ++d) c *= split;
+        make_grid(g.box, nd, reach / split, c, grid);
+        grid_span = split;
+        for (int d = 0; d < nd; ++d)
+            if (grid.nc[d] != 1 && grid.nc[d] < 2 * split + 1) grid_span = 1;
+    }
+    if (grid_span == 1) make_grid(g.box, nd, reach, cap, grid);
     const size_t total = (size_t)grid.total;
     if (total + 1 > cell_cap) {
         if (cell_count) {
@@ -457,6 +468,7 @@ void Engine::setup(const rcp_run_config& c, bool hosted)
         throw ArgError("rcp: exact_reductions needs neighbour lists (neighbor_mode 0 or 1)");
     if (c.neighbor_mode == 3 && c.ndim != 2 && c.ndim != 3)
         throw ArgError("rcp: tiled steps (neighbor_mode 3) need ndim 2 or 3");
+    if (c.tile_split > 3) throw ArgError("rcp: tile_split must be 0 (table), 1, 2 or 3");
     N_ = c.N;
     nd_ = c.ndim;
     opt_ = c;
@@ -902,8 +914,9 @@ void Engine::tile_rebin(double skin)
     // the binning stays valid while every particle has moved less than half
     // the skin (diameter growth and box stretch included).
     begin_timer(0);
-    st_->rebin(geo_, dmax_now() + skin);
-    if (!plan_tiles((int)nd_, st_->grid, (long long)N_, has_fixed_ ? 1 : 0,
+    const int split = opt_.tile_split > 0 ? (int)opt_.tile_split : tuned_tile_split((int)nd_);
+    st_->rebin(geo_, dmax_now() + skin, split);
+    if (!plan_tiles((int)nd_, st_->grid, st_->grid_span, (long long)N_, has_fixed_ ? 1 : 0,
                     (int)opt_.tile_particles, tile_plan_))
         throw RtError("rcp: no tile plan for this grid");
     if (skin > 0.0)

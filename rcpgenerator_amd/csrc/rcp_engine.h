@@ -58,8 +58,11 @@ public:
 
     // upload reference-ordered positions / diameters / mask (tags = identity)
     void upload(const double* x, const double* D0, const int8_t* fixed);
-    // re-bin into cell order for cells at least `reach` wide
-    void rebin(const Geometry& g, double reach);
+    // re-bin into cell order for cells at least `reach` / `split` wide; an
+    // axis must then hold one cell or at least 2 * split + 1 (the stencil of
+    // +-split cells covers it once), else the grid falls back to split 1.
+    // grid_span records the split taken.
+    void rebin(const Geometry& g, double reach, int split = 1);
     // re-bin by size class into class-major cell order
     void rebin_multi(const Geometry& g, const MultiGrid& mg, double kappa);
     // rows of the current binning: rc = (Di + Dj)*0.5 + t_shell
@@ -81,6 +84,7 @@ public:
     bool has_fixed_array, exact;
     hipStream_t s;
     CellGrid grid;
+    int grid_span = 1;
     uint32_t nnz = 0;
     bool binned = false, inv_tag_valid = true;
     size_t partial_blocks = 0;

@@ -99,7 +99,9 @@ struct StepArgs {
     // tiled step (launch_step_tile): plan from plan_tiles; dmax = largest
     // current diameter, pad = largest drift of a particle from its binned
     // cell (half the skin)
-    int tile_len = 0, tiles_per_row = 0, hc_cap = 0, h_cap = 0, list_cap = 0;
+    // cells are at least 1/tile_span of the reach wide: stencils of
+    // +-tile_span cells along every axis holding more than one cell
+    int tile_len = 0, tiles_per_row = 0, hc_cap = 0, h_cap = 0, list_cap = 0, tile_span = 1;
     double dmax = 0.0, pad = 0.0;
 };
 
@@ -112,13 +114,15 @@ struct StepArgs {
 // wide as the contact reach at binning time plus twice the largest
 // displacement since, so one binning can serve several steps.
 struct TilePlan {
-    int block = 0, tile_len = 0, tiles_per_row = 0, hc_cap = 0, h_cap = 0, list_cap = 0;
+    int block = 0, tile_len = 0, tiles_per_row = 0, hc_cap = 0, h_cap = 0, list_cap = 0, span = 1;
     long long tiles = 0, blocks = 0;   // tiles; persistent workgroups
     size_t lds_bytes = 0;
 };
 // false when the tiled step does not handle this shape (ndim not 2 or 3).
-// per_tile: target particles per tile (0: the autotuned table)
-bool plan_tiles(int ndim, const CellGrid& grid, long long N, int use_rk, int per_tile,
+// span: the grid's cells are at least 1/span of the reach wide (every axis
+// holds one cell or at least 2 * span + 1); per_tile: target particles per
+// tile (0: the autotuned table)
+bool plan_tiles(int ndim, const CellGrid& grid, int span, long long N, int use_rk, int per_tile,
                 TilePlan& p);
 int launch_step_tile(const StepArgs& a, const TilePlan& p, hipStream_t s,
                      hipEvent_t mark = nullptr);

@@ -606,20 +606,26 @@ __device__ __forceinline__ int find_cell(const uint32_t* off, int lo, int hi, ui
 }
 
 // Tile geometry shared by the staging and the per-particle passes.
+// Cells are at least 1/S of the reach wide (S = a.tile_span); the stencil
+// spans S cells either side on every axis of 2 S + 1 or more cells (rspan
+// = 2 S + 1) and the whole axis otherwise (one cell, rspan = 1).
 template <int ND>
 struct TileGeo {
-    int nx, span_x, cx_lo, ncx, kx, nrows, ro_c;
+    int S, nx, span_x, cx_lo, ncx, kx, nrows, ro_c;
     int rc[ND], rspan[ND];
     double e[ND];   // cell edge per axis (box / cells)
 };
 
 // Forces, pair terms and the ADAM tail of the tile's own particles, from
 // the staged halo.  PLAIN tiles (no halo cell across the periodic boundary,
-// seven or more cells on every axis) have |x_j - x_i| < L/2 for every pair
-// in a stencil, so the minimum image is the identity and the plain
-// difference is the reference's arithmetic bit for bit; they also skip
-// the stencil cells a particle cannot reach ((Di + Dmax)/2 plus drift).
-// Other tiles take the full minimum-image form over the whole stencil.
+// 4 S + 3 or more cells on every axis) have |x_j - x_i| < L/2 for every pair
+// in a stencil: the stencil spans (S + 1) cells of width e and binned
+// particles drift less than half the skin, which is below S e / 2, so
+// |x_j - x_i| < (2 S + 1) e <= L / 2.  The minimum image is then the
+// identity and the plain difference is the reference's arithmetic bit for
+// bit.  PLAIN tiles also skip the stencil cells a particle cannot reach
+// ((Di + Dmax)/2 plus drift).  Other tiles take the full minimum-image form
+// over the whole stencil.
 template <int ND, int BLOCK, bool PLAIN>
 __device__ __forceinline__ void tile_particles_lds(
     const StepArgs& a, const TileGeo<ND>& t, const uint32_t* hc_off, const double* sx,
@@ -638,11 +644,16 @@ __device__ __forceinline__ void tile_particles_lds(
         int k0 = kc - t.span_x, k1 = kc + t.span_x;
         double R = 0.0;
         if (PLAIN) {
-            // halo cell k along x is global cell cx_lo - 1 + k: [cx*e, (cx+1)*e)
+            // halo cell k along x is global cell cx_lo - span_x + k:
+            // [cx*e, (cx+1)*e); cells wholly beyond x -+ R are skipped
             R = 0.5 * (o.D + a.dmax) + a.pad;
-            const double left = (double)(t.cx_lo - 1 + kc) * t.e[0];
-            if (o.x[0] - R > left * (1.0 + tol) + tol * t.e[0]) k0 = kc;
-            if (o.x[0] + R < (left + t.e[0]) * (1.0 - tol) - tol * t.e[0]) k1 = kc;
+            const double left = (double)(t.cx_lo - t.span_x + kc) * t.e[0];
+            while (k0 < kc &&
+                   o.x[0] - R > (left + (double)(k0 - kc + 1) * t.e[0]) * (1.0 + tol) + tol * t.e[0])
+                ++k0;
+            while (k1 > kc &&
+                   o.x[0] + R < (left + (double)(k1 - kc) * t.e[0]) * (1.0 - tol) - tol * t.e[0])
+                --k1;
         }
         double F[ND];
 #pragma unroll
@@ -661,17 +672,18 @@ __device__ __forceinline__ void tile_particles_lds(
         int n = 0;
         for (int ro = 0; ro < t.nrows; ++ro) {
             if (PLAIN) {
-                // rows out of reach along y (and z) are skipped
+                // rows out of reach along y (and z) are skipped (PLAIN
+                // tiles span 2 S + 1 rows on every axis)
                 bool reach = true;
                 int tt = ro;
 #pragma unroll
                 for (int d = 1; d < ND; ++d) {
-                    const int kd = tt % 3;
-                    tt /= 3;
-                    const double lo = (double)(t.rc[d] + kd - 1) * t.e[d];
-                    if (kd == 0 && o.x[d] - R > (lo + t.e[d]) * (1.0 + tol) + tol * t.e[d])
+                    const int off = tt % t.rspan[d] - t.S;
+                    tt /= t.rspan[d];
+                    const double lo = (double)(t.rc[d] + off) * t.e[d];
+                    if (off < 0 && o.x[d] - R > (lo + t.e[d]) * (1.0 + tol) + tol * t.e[d])
                         reach = false;
-                    if (kd == 2 && o.x[d] + R < lo * (1.0 - tol) - tol * t.e[d]) reach = false;
+                    if (off > 0 && o.x[d] + R < lo * (1.0 - tol) - tol * t.e[d]) reach = false;
                 }
                 if (!reach) continue;
             }
@@ -846,8 +858,9 @@ __global__ __launch_bounds__(BLOCK) void k_step_tile(StepArgs a, long long ntile
     for (int k = 0; k < S_NUM; ++k) acc[k] = 0.0;
 
     TileGeo<ND> t;
+    t.S = a.tile_span;
     t.nx = a.grid.nc[0];
-    t.span_x = t.nx >= 3 ? 1 : 0;
+    t.span_x = t.nx >= 2 * t.S + 1 ? t.S : 0;
 #pragma unroll
     for (int d = 0; d < ND; ++d) t.e[d] = a.g.box[d] / (double)a.grid.nc[d];
 
@@ -879,8 +892,8 @@ __global__ __launch_bounds__(BLOCK) void k_step_tile(StepArgs a, long long ntile
             for (int d = 1; d < ND; ++d) {
                 t.rc[d] = r % a.grid.nc[d];
                 r /= a.grid.nc[d];
-                t.rspan[d] = a.grid.nc[d] >= 3 ? 3 : 1;
-                if (t.rspan[d] == 3) t.ro_c += t.nrows;
+                t.rspan[d] = a.grid.nc[d] >= 2 * t.S + 1 ? 2 * t.S + 1 : 1;
+                if (t.rspan[d] > 1) t.ro_c += t.S * t.nrows;
                 t.nrows *= t.rspan[d];
             }
         }
@@ -895,9 +908,9 @@ __global__ __launch_bounds__(BLOCK) void k_step_tile(StepArgs a, long long ntile
 #pragma unroll
             for (int d = 1; d < ND; ++d) {
                 int cd = t.rc[d];
-                if (t.rspan[d] == 3) {
-                    cd += tt % 3 - 1;
-                    tt /= 3;
+                if (t.rspan[d] > 1) {
+                    cd += tt % t.rspan[d] - t.S;
+                    tt /= t.rspan[d];
                     if (cd < 0) cd += a.grid.nc[d];
                     if (cd >= a.grid.nc[d]) cd -= a.grid.nc[d];
                 }
@@ -932,13 +945,15 @@ __global__ __launch_bounds__(BLOCK) void k_step_tile(StepArgs a, long long ntile
                 stag[i] = a.tag[p];
             }
             __syncthreads();
-            // no halo cell across the periodic boundary, seven or more cells
-            // on every axis: the minimum image is the identity in this tile
-            bool plain = t.span_x && t.cx_lo >= 1 && t.cx_lo + t.ncx <= t.nx - 1 && t.nx >= 7;
+            // no halo cell across the periodic boundary, 4 S + 3 or more
+            // cells on every axis: the minimum image is the identity here
+            const int S = t.S;
+            bool plain = t.span_x && t.cx_lo >= S && t.cx_lo + t.ncx <= t.nx - S &&
+                         t.nx >= 4 * S + 3;
 #pragma unroll
             for (int d = 1; d < ND; ++d)
-                plain = plain && a.grid.nc[d] >= 7 && t.rc[d] >= 1 &&
-                        t.rc[d] <= a.grid.nc[d] - 2;
+                plain = plain && a.grid.nc[d] >= 4 * S + 3 && t.rc[d] >= S &&
+                        t.rc[d] <= a.grid.nc[d] - 1 - S;
             if (plain)
                 tile_particles_lds<ND, BLOCK, true>(a, t, hc_off, sx, sD, srk, stag, lists + tid,
                                                     own_h0, own_lo, own_hi, q_own, acc);
@@ -2422,16 +2437,20 @@ long long tile_grid(int ndim, size_t lds_bytes)
     return c_val;
 }
 
-bool plan_tiles(int ndim, const CellGrid& grid, long long N, int use_rk, int per_tile,
+bool plan_tiles(int ndim, const CellGrid& grid, int span, long long N, int use_rk, int per_tile,
                 TilePlan& p)
 {
     if (ndim != 2 && ndim != 3) return false;
     if (per_tile <= 0) per_tile = tuned_tile_particles(ndim);
+    if (span < 1) return false;
+    for (int d = 0; d < ndim; ++d)
+        if (grid.nc[d] != 1 && grid.nc[d] < 2 * span + 1) return false;
     p.block = kTileBlock;
+    p.span = span;
     const int nx = grid.nc[0];
-    const int span_x = nx >= 3 ? 1 : 0;
+    const int span_x = nx >= 2 * span + 1 ? span : 0;
     int nrows = 1;
-    for (int d = 1; d < ndim; ++d) nrows *= grid.nc[d] >= 3 ? 3 : 1;
+    for (int d = 1; d < ndim; ++d) nrows *= grid.nc[d] >= 2 * span + 1 ? 2 * span + 1 : 1;
     const double rho = (double)N / (double)grid.total;   // particles per cell
     const int hc_max = 1024;
     const int t_max = hc_max / nrows - 2 * span_x;
@@ -2471,6 +2490,7 @@ int launch_step_tile(const StepArgs& a0, const TilePlan& p, hipStream_t s, hipEv
     a.hc_cap = p.hc_cap;
     a.h_cap = p.h_cap;
     a.list_cap = p.list_cap;
+    a.tile_span = p.span;
     const unsigned g = (unsigned)p.blocks;
     switch (a.g.ndim) {
         case 2: k_step_tile<2, kTileBlock><<<g, kTileBlock, p.lds_bytes, s>>>(a, p.tiles); break;

@@ -142,6 +142,9 @@ void parse_run(const py::dict& input, const py::dict& config, const py::dict& op
     size_t tile_particles = 0;
     take(options, "tile_particles", tile_particles);
     c.tile_particles = tile_particles;
+    size_t tile_split = 0;
+    take(options, "tile_split", tile_split);
+    c.tile_split = tile_split;
     take(options, "fix_diameter", fix_diameter);
     take(options, "reorder_interval", reorder_interval);
     c.max_steps = max_steps;

@@ -6,20 +6,30 @@ from rcpgenerator_amd import autotune as A
 
 
 def _r(**ms):
-    return {int(k[1:]): {"kernel_ms": v} for k, v in ms.items()}
+    """Shapes named s<split>t<particles>: cost in ms."""
+    out = {}
+    for k, v in ms.items():
+        sp, t = k[1:].split("t")
+        out[(int(sp), int(t))] = {"cost_ms": v}
+    return out
 
 
 def test_default_kept_within_margin():
-    assert A.choose(_r(t64=1.0, t96=0.98, t128=1.0)) == 128          # 2 % faster: noise
-    assert A.choose(_r(t64=0.80, t96=0.95, t128=1.0)) == 64           # 20 % faster
-    assert A.choose(_r(t64=1.2, t128=1.1, t256=1.3)) == 128
+    assert A.choose(_r(s1t64=1.0, s1t96=0.98, s1t128=1.0)) == (1, 128)    # 2 % faster: noise
+    assert A.choose(_r(s1t64=0.80, s1t96=0.95, s1t128=1.0)) == (1, 64)     # 20 % faster
+    assert A.choose(_r(s1t64=1.2, s1t128=1.1, s1t256=1.3)) == (1, 128)
+    assert A.choose(_r(s1t128=1.0, s2t128=0.9, s2t96=0.95)) == (2, 128)
+    assert A.choose(_r(s1t128=1.0, s2t128=0.99)) == (1, 128)
 
 
 def test_header_table_matches_its_recorded_sweep():
     p = os.path.join(os.path.dirname(A.__file__), "csrc", "rcp_tile_table.h")
     text = open(p).read()
-    table = {int(a): int(b) for a, b in re.findall(r"\{(\d), (\d+)\},", text)}
+    table = {int(a): (int(c), int(b)) for a, b, c in re.findall(r"\{(\d), (\d+), (\d)\},", text)}
     for nd in (2, 3):
-        line = re.search(rf"//   {nd}-D: (.*)", text).group(1)
-        r = {int(t): {"kernel_ms": float(v)} for t, v in re.findall(r"(\d+): ([\d.]+)", line)}
+        r = {}
+        for sp, line in re.findall(rf"//   {nd}-D split (\d): (.*)", text):
+            for t, v in re.findall(r"(\d+): ([\d.]+)", line):
+                r[(int(sp), int(t))] = {"cost_ms": float(v)}
+        assert r, nd
         assert table[nd] == A.choose(r)

@@ -319,20 +319,27 @@ def test_neighbour_modes_agree(gpu, nd, phi, cluster):
     the neighbour lists, the per-step cell scan and the tiled kernel must
     give bit-identical trajectories.  Dense cases overflow the contact
     buffers (selection path); clustered cases overflow the tiles' LDS
-    staging (global-memory path)."""
+    staging (global-memory path).  The tiled kernel runs with cells of the
+    full reach and of 1/2 and 1/3 of it (stencils of +-2 and +-3 cells, the
+    reach-limited scan of interior tiles included)."""
     rng = np.random.default_rng(int(10 * phi) + nd + 100 * cluster)
     N = 20000 if phi < 3 else 3000
     x, D, box = (clustered_config if cluster else random_config)(rng, N, nd, phi)
     out = {}
-    for mode in (1, 2, 3):
-        e = gpu.Engine(x, D, box=box, max_steps=40, fix_diameter=True, neighbor_mode=mode)
+    for mode, split in ((1, 0), (2, 0), (3, 1), (3, 2), (3, 3)):
+        e = gpu.Engine(x, D, box=box, max_steps=40, fix_diameter=True, neighbor_mode=mode,
+                       tile_split=split)
         e.run(0)
-        out[mode] = (e.result(), e.stats())
-    (a, sa), (b, sb), (c, sc) = out[1], out[2], out[3]
+        out[(mode, split)] = (e.result(), e.stats())
+    (a, sa), (b, sb), (c, sc) = out[(1, 0)], out[(2, 0)], out[(3, 1)]
     assert sb["direct_steps"] == 40 and sa["direct_steps"] == 0
     assert sc["tile_steps"] == 40
     assert same(a["positions"], b["positions"])
     assert same(a["positions"], c["positions"])
+    for split in (2, 3):
+        r, st = out[(3, split)]
+        assert st["tile_steps"] == 40, split
+        assert same(a["positions"], r["positions"]), split
     if phi > 3:
         assert sb["overflow_steps"] > 0
         if nd == 2:
