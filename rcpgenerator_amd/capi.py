@@ -34,7 +34,7 @@ SYMBOLS = (
     "rcp_adam_positions", "rcp_reorder_perm", "rcp_scale_diameters", "rcp_init_config_init",
     "rcp_initialize_particles", "rcp_mlp_create", "rcp_mlp_destroy", "rcp_mlp_num_params",
     "rcp_mlp_get_params", "rcp_mlp_set_params", "rcp_mlp_predict", "rcp_mlp_gradients",
-    "rcp_mlp_train",
+    "rcp_mlp_train", "rcp_set_team_lanes",
 )
 
 
@@ -152,6 +152,7 @@ def lib():
         L.rcp_batch_set_progress.argtypes = [BatchProgressFn, C.c_void_p, C.c_uint64]
         L.rcp_batch_last_stats.argtypes = [C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                            C.POINTER(C.c_double)]
+        L.rcp_set_team_lanes.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -395,6 +396,12 @@ def run_batch(runs, threads: int = 0, pool: bool = False, diameters: bool = Fals
     if rc != RCP_OK and all(r is None or r.get("rc") == RCP_OK for r in out):
         _check(rc)
     return out, {"rounds": rounds.value, "packing_steps": steps.value, "device_ms": ms.value}
+
+
+def set_team_lanes(lanes: int) -> None:
+    """Lanes per particle of the size-class step: 0 = automatic (default),
+    1, 2, 4 or 8 forced (a measurement hook: results do not depend on it)."""
+    _check(lib().rcp_set_team_lanes(int(lanes)))
 
 
 def initialize(config: dict):

@@ -396,6 +396,16 @@ int rcp_batch_last_stats(uint64_t* rounds, uint64_t* packing_steps, double* devi
     return RCP_OK;
 }
 
+int rcp_set_team_lanes(int lanes)
+{
+    if (lanes != 0 && lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8) {
+        rcp::g_last_error = "rcp: team lanes must be 0 (automatic), 1, 2, 4 or 8";
+        return RCP_E_ARG;
+    }
+    rcp::set_team_lanes(lanes);
+    return RCP_OK;
+}
+
 // --------------------------------------------------------- kernel level
 
 int rcp_forces(size_t N, size_t ndim, const double* x, const double* D, const double* box,

@@ -131,6 +131,15 @@ int launch_step_tile(const StepArgs& a, const TilePlan& p, hipStream_t s,
 size_t step_partial_blocks(size_t n);
 // Each step launcher runs the step kernel and then k_fold; when `mark` is
 // given it is recorded between the two (timing of the step kernel alone).
+// Lanes per particle of the size-class step (per-thread kernel or teams of
+// 2, 4 or 8 lanes; 4 at most from five dimensions up): the smallest team
+// that gives the launch kTeamThreads threads (about eight waves on every
+// SIMD of the chip).  Results do not depend on it.  set_team_lanes(n > 0)
+// forces n lanes (rounded down to a power of two within those limits; for
+// measurements), 0 restores the rule.
+constexpr size_t kTeamThreads = size_t(1) << 18;
+int team_lanes_log2(size_t particles, int ndim);
+void set_team_lanes(int lanes);
 int launch_step_list(const StepArgs& a, hipStream_t s, hipEvent_t mark = nullptr);
 int launch_step_direct(const StepArgs& a, hipStream_t s, hipEvent_t mark = nullptr);
 // List step whose scalar terms are then folded in the reference's

@@ -1007,9 +1007,14 @@ struct StencilBox {
     }
 };
 
+// Cells of every class stencil of a particle at x with diameter D that a
+// contact can lie in: fn(q0, q1) with the cell's working-order range.  The
+// caller takes the stencil cells first, first + step, ... of every class
+// (lanes of a team split the cells this way).
 template <int ND, class Fn>
-__device__ __forceinline__ void for_each_class_candidate(const StepArgs& a, const MultiGrid& mg,
-                                                         const double* x, double D, Fn&& fn)
+__device__ __forceinline__ void for_each_class_cell(const StepArgs& a, const MultiGrid& mg,
+                                                    const double* x, double D, int first,
+                                                    int step, Fn&& fn)
 {
     for (int b = 0; b < mg.C; ++b) {
         const CellGrid& gr = mg.grid[b];
@@ -1043,7 +1048,7 @@ __device__ __forceinline__ void for_each_class_candidate(const StepArgs& a, cons
         int ncell = 1;
 #pragma unroll
         for (int d = 0; d < ND; ++d) ncell *= cnt[d];
-        for (int s = 0; s < ncell; ++s) {
+        for (int s = first; s < ncell; s += step) {
             int t = s;
             long long cc = 0, stride = 1;
             double far2 = 0.0;
@@ -1059,14 +1064,57 @@ __device__ __forceinline__ void for_each_class_candidate(const StepArgs& a, cons
             }
             if (far2 > R2) continue;   // no point of the cell within R
             cc += mg.base[b];
-            const uint32_t q0 = a.cell_start[cc], q1 = a.cell_start[cc + 1];
-            for (uint32_t qj = q0; qj < q1; ++qj) fn(qj);
+            fn(a.cell_start[cc], a.cell_start[cc + 1]);
         }
     }
 }
 
+template <int ND, class Fn>
+__device__ __forceinline__ void for_each_class_candidate(const StepArgs& a, const MultiGrid& mg,
+                                                         const double* x, double D, Fn&& fn)
+{
+    for_each_class_cell<ND>(a, mg, x, D, 0, 1, [&](uint32_t q0, uint32_t q1) {
+        for (uint32_t qj = q0; qj < q1; ++qj) fn(qj);
+    });
+}
+
 constexpr int kMultiBlock = 128;
 constexpr int kMultiCap = 24;
+
+// Orders the LDS accesses of one wavefront (the waves of a workgroup work
+// on their own LDS regions and never wait for each other).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// More contacts than a buffer holds: the contacts of particle q in tag
+// order by repeated scans of its stencils (each scan takes the next tag).
+template <int ND>
+__device__ __forceinline__ void multi_contacts_by_selection(const StepArgs& a, const MultiGrid& mg,
+                                                            uint32_t q, const Own<ND>& o,
+                                                            double (&F)[ND], double (&acc)[S_NUM])
+{
+    acc[S_OVERFLOW] += 1.0;
+    long long last = -1;
+    while (true) {
+        unsigned long long best = ~0ull;
+        for_each_class_candidate<ND>(a, mg, o.x, o.D, [&](uint32_t qj) {
+            if (qj == q) return;
+            const uint32_t tj = a.tag[qj];
+            if ((long long)tj <= last) return;
+            const unsigned long long key = ((unsigned long long)tj << 32) | qj;
+            if (key >= best) return;
+            if (overlaps<ND>(a, o, qj)) best = key;
+        });
+        if (best == ~0ull) break;
+        contact<ND>(a, o, (uint32_t)(best & 0xffffffffull), (uint32_t)(best >> 32), F, acc,
+                    nullptr);
+        last = (long long)(best >> 32);
+    }
+}
 
 // Particle q of the per-thread size-class step (thread tid of its
 // workgroup, buf its contact buffer): forces, pair and wall terms into acc.
@@ -1106,23 +1154,7 @@ __device__ __forceinline__ void multi_particle(const StepArgs& a, const MultiGri
                 contact<ND>(a, o, j, a.tag[j], F, acc, nullptr);
             }
         } else {
-            acc[S_OVERFLOW] += 1.0;
-            long long last = -1;
-            while (true) {
-                unsigned long long best = ~0ull;
-                for_each_class_candidate<ND>(a, mg, o.x, o.D, [&](uint32_t qj) {
-                    if (qj == (uint32_t)q) return;
-                    const uint32_t tj = a.tag[qj];
-                    if ((long long)tj <= last) return;
-                    const unsigned long long key = ((unsigned long long)tj << 32) | qj;
-                    if (key >= best) return;
-                    if (overlaps<ND>(a, o, qj)) best = key;
-                });
-                if (best == ~0ull) break;
-                contact<ND>(a, o, (uint32_t)(best & 0xffffffffull), (uint32_t)(best >> 32), F, acc,
-                            nullptr);
-                last = (long long)(best >> 32);
-            }
+            multi_contacts_by_selection<ND>(a, mg, (uint32_t)q, o, F, acc);
         }
         particle_tail<ND>(a, q, o, F, acc);
     }
@@ -1141,6 +1173,139 @@ __global__ __launch_bounds__(kMultiBlock) void k_step_multi(StepArgs a, MultiGri
     finish_grid<kMultiBlock>(a, blk);
 }
 
+// Team variant of the per-thread size-class step, for launches too small to
+// fill the chip with one thread per particle: a team of 2^lg lanes (2, 4 or
+// 8, within one wavefront) per particle, the same kMultiBlock particles per
+// workgroup.  The lanes split the stencil cells and append contacts as
+// (tag << 32 | position) keys to the team's LDS buffer; each lane ranks its
+// keys among the team's (keys are distinct), and the team's first lane
+// adds the pair terms in tag order, then the wall terms and ADAM -- the
+// arithmetic of multi_particle.  The first lanes' sums then go through the
+// reduction tree of block_reduce<kMultiBlock> over the particles of the
+// workgroup, so the partial rows, and every result, are those of
+// k_step_multi bit for bit whatever the team size.
+constexpr int kTeamMaxLanes = 8;
+// Teams of 8 lanes make 1024-thread workgroups, which hold a thread to 128
+// registers; from five dimensions up that spills, so teams there stop at 4.
+__host__ __device__ constexpr int team_max_lanes(int nd)
+{
+    return nd <= 4 ? kTeamMaxLanes : kTeamMaxLanes / 2;
+}
+
+struct TeamLds {
+    union {
+        unsigned long long keys[kMultiBlock][kMultiCap];
+        double sums[kMultiBlock][S_NUM];   // after the contact phase
+    };
+    uint8_t rank_of[kMultiBlock][kMultiCap];   // slot of the k-th smallest key
+    int cnt[kMultiBlock];
+    double wave_tot[kMultiBlock / kWave][S_NUM];
+};
+
+template <int ND>
+__device__ __forceinline__ void team_particle(const StepArgs& a, const MultiGrid& mg, int q,
+                                              int team, int tl, int lanes, TeamLds& T,
+                                              double (&acc)[S_NUM])
+{
+    const uint32_t n_wide = mg.wide > 0 ? a.cell_start[mg.base[mg.wide]] : 0u;
+    const bool active = q < a.g.N && (uint32_t)q >= n_wide;   // uniform across the team
+    Own<ND> o;
+    if (active) load_own<ND>(a, q, o);
+    if (tl == 0) T.cnt[team] = 0;
+    wave_sync();
+    if (active) {
+        for_each_class_cell<ND>(a, mg, o.x, o.D, tl, lanes, [&](uint32_t q0, uint32_t q1) {
+            for (uint32_t qj = q0; qj < q1; ++qj) {
+                if (qj == (uint32_t)q || !overlaps<ND>(a, o, qj)) continue;
+                const int slot = atomicAdd(&T.cnt[team], 1);
+                if (slot < kMultiCap)
+                    T.keys[team][slot] = ((unsigned long long)a.tag[qj] << 32) | qj;
+            }
+        });
+    }
+    wave_sync();
+    const int n = active ? T.cnt[team] : 0;
+    if (n <= kMultiCap) {
+        for (int p = tl; p < n; p += lanes) {
+            const unsigned long long kp = T.keys[team][p];
+            int r = 0;
+            for (int k = 0; k < n; ++k) r += T.keys[team][k] < kp ? 1 : 0;
+            T.rank_of[team][r] = (uint8_t)p;
+        }
+    }
+    wave_sync();
+    if (active && tl == 0) {
+        double F[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) F[d] = 0.0;
+        if (n <= kMultiCap) {
+            for (int p = 0; p < n; ++p) {
+                const unsigned long long key = T.keys[team][T.rank_of[team][p]];
+                contact<ND>(a, o, (uint32_t)(key & 0xffffffffull), (uint32_t)(key >> 32), F, acc,
+                            nullptr);
+            }
+        } else {
+            multi_contacts_by_selection<ND>(a, mg, (uint32_t)q, o, F, acc);
+        }
+        particle_tail<ND>(a, q, o, F, acc);
+    }
+}
+
+// Workgroup totals of the team step: the first lane of team t holds the
+// sums of particle t; thread t of the first kMultiBlock threads takes them
+// and the tree of block_reduce<kMultiBlock> follows.  Thread 0 receives the
+// totals.
+__device__ __forceinline__ void team_reduce(TeamLds& T, int team, int tl,
+                                            const double (&acc)[S_NUM], double* out)
+{
+    __syncthreads();   // every team is past its keys (sums alias them)
+    if (tl == 0) {
+#pragma unroll
+        for (int k = 0; k < S_NUM; ++k) T.sums[team][k] = acc[k];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    if (threadIdx.x < kMultiBlock) {
+        double v[S_NUM];
+#pragma unroll
+        for (int k = 0; k < S_NUM; ++k) v[k] = T.sums[threadIdx.x][k];
+#pragma unroll
+        for (int k = 0; k < S_NUM; ++k) v[k] = is_max_slot(k) ? wave_max(v[k]) : wave_sum(v[k]);
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < S_NUM; ++k) T.wave_tot[w][k] = v[k];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < S_NUM; ++k) {
+            double t = T.wave_tot[0][k];
+            for (int u = 1; u < kMultiBlock / kWave; ++u) {
+                if (is_max_slot(k)) t = (T.wave_tot[u][k] > t) ? T.wave_tot[u][k] : t;
+                else t += T.wave_tot[u][k];
+            }
+            out[k] = t;
+        }
+    }
+}
+
+template <int ND>
+__global__ __launch_bounds__(kMultiBlock * team_max_lanes(ND))
+void k_step_team(StepArgs a, MultiGrid mg, int lg)
+{
+    __shared__ TeamLds T;
+    const int lanes = 1 << lg;
+    const int team = threadIdx.x >> lg, tl = threadIdx.x & (lanes - 1);
+    double acc[S_NUM];
+#pragma unroll
+    for (int k = 0; k < S_NUM; ++k) acc[k] = 0.0;
+    team_particle<ND>(a, mg, (int)(blockIdx.x * kMultiBlock) + team, team, tl, lanes, T, acc);
+    double blk[S_NUM];
+    team_reduce(T, team, tl, acc, blk);
+    finish_grid<kMultiBlock>(a, blk);
+}
+
 // One wavefront per particle of the largest classes (a big particle can
 // touch hundreds of small ones).  The lanes split the stencil cells and
 // append contacts as (tag << 32 | position) keys to LDS; a wave-wide bitonic
@@ -1148,14 +1313,6 @@ __global__ __launch_bounds__(kMultiBlock) void k_step_multi(StepArgs a, MultiGri
 // time in parallel and lane 0 adds them in tag order, so the sums round as
 // in the per-thread kernels.  Persistent: wave w of the grid takes particles
 // w, w + waves, ...
-// Orders the LDS accesses of one wavefront (the waves of a workgroup work
-// on their own LDS regions and never wait for each other).
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 constexpr int kWideWaves = 4;                  // wavefronts per workgroup
 constexpr int kWideGrid = 1024;                // persistent workgroups
@@ -2270,6 +2427,28 @@ __global__ __launch_bounds__(kMultiBlock) void kb_step_multi(const BatchPack* __
 }
 
 template <int ND>
+__global__ __launch_bounds__(kMultiBlock * team_max_lanes(ND))
+void kb_step_team(const BatchPack* __restrict__ packs, BatchTable t, int lg)
+{
+    __shared__ TeamLds T;
+    const int e = table_entry(t, blockIdx.x);
+    const BatchPack& P = packs[table_item(t, e)];
+    const unsigned lb = blockIdx.x - t.prefix[e];
+    const int lanes = 1 << lg;
+    const int team = threadIdx.x >> lg, tl = threadIdx.x & (lanes - 1);
+    double acc[S_NUM];
+#pragma unroll
+    for (int k = 0; k < S_NUM; ++k) acc[k] = 0.0;
+    team_particle<ND>(P.a, P.mg, (int)(lb * kMultiBlock) + team, team, tl, lanes, T, acc);
+    double blk[S_NUM];
+    team_reduce(T, team, tl, acc, blk);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < S_NUM; ++k) P.a.partials[(size_t)lb * S_NUM + k] = blk[k];
+    }
+}
+
+template <int ND>
 __global__ __launch_bounds__(kWideBlock) __attribute__((amdgpu_waves_per_eu(ND <= 2 ? 3 : 1)))
 void kb_step_wide(const BatchPack* __restrict__ packs, BatchTable t)
 {
@@ -2517,12 +2696,39 @@ size_t multi_partial_rows(size_t n)
     return grid_for(n, kMultiBlock) + (size_t)kWideGrid;
 }
 
+namespace {
+int g_team_lanes = 0;   // 0: team_lanes_log2's rule
+}
+
+void set_team_lanes(int lanes)
+{
+    g_team_lanes = lanes;
+}
+
+int team_lanes_log2(size_t particles, int ndim)
+{
+    const int most = team_max_lanes(ndim);
+    int lg = 0;
+    if (g_team_lanes > 0) {
+        while ((2 << lg) <= g_team_lanes && (1 << lg) < most) ++lg;
+        return lg;
+    }
+    while ((1 << lg) < most && particles * (size_t)(1 << lg) < kTeamThreads) ++lg;
+    return lg;
+}
+
 int launch_step_multi(const StepArgs& a0, const MultiGrid& mg, hipStream_t s, hipEvent_t mark)
 {
     if (a0.g.N <= 0) return 0;
     StepArgs a = a0;
     const unsigned g = grid_for(a.g.N, kMultiBlock);
-    RCP_DISPATCH_ND(a.g.ndim, k_step_multi, <<<g, kMultiBlock, 0, s>>>(a, mg));
+    const int lg = team_lanes_log2((size_t)a.g.N, a.g.ndim);
+    if (lg > 0) {
+        RCP_DISPATCH_ND(a.g.ndim, k_step_team,
+                        <<<g, (unsigned)(kMultiBlock << lg), 0, s>>>(a, mg, lg));
+    } else {
+        RCP_DISPATCH_ND(a.g.ndim, k_step_multi, <<<g, kMultiBlock, 0, s>>>(a, mg));
+    }
     unsigned rows = g;
     if (mg.wide > 0) {
         a.tile_len = (int)g;   // first partial row of k_step_wide
@@ -2848,8 +3054,13 @@ int launch_batch_step(int ndim, const BatchPack* packs, int npacks, const BatchT
                       const BatchTable& wide, double* results, hipStream_t s, hipEvent_t mark)
 {
     if (npacks <= 0) return 0;
-    if (multi.blocks > 0)
+    const int lg = team_lanes_log2((size_t)multi.blocks * kMultiBlock, ndim);
+    if (multi.blocks > 0 && lg > 0) {
+        RCP_DISPATCH_ND(ndim, kb_step_team,
+                        <<<multi.blocks, (unsigned)(kMultiBlock << lg), 0, s>>>(packs, multi, lg));
+    } else if (multi.blocks > 0) {
         RCP_DISPATCH_ND(ndim, kb_step_multi, <<<multi.blocks, kMultiBlock, 0, s>>>(packs, multi));
+    }
     if (wide.blocks > 0)
         RCP_DISPATCH_ND(ndim, kb_step_wide, <<<wide.blocks, kWideBlock, 0, s>>>(packs, wide));
     if (mark && hipEventRecord(mark, s) != hipSuccess) return 1;

@@ -515,6 +515,52 @@ def _assert_same_run(r, s):
     assert r["mu_flag_history"] == s["mu_flag_history"]
 
 
+@pytest.mark.parametrize("nd,N,ratio", [(2, 6000, 10.0), (3, 8000, 10.0), (5, 2000, 4.0)])
+def test_team_lanes_do_not_change_results(gpu, nd, N, ratio):
+    """The size-class step with 1 (per-thread kernel), 2, 4 or 8 lanes per
+    particle: identical runs, scalars and diameters fed back (no fixed
+    diameters), bit for bit.  Five dimensions stop at 4 lanes."""
+    from rcpgenerator_amd import capi
+    rng = np.random.default_rng(800 + nd)
+    x, D, box = powerlaw_config(rng, N, nd, 0.3 if nd < 5 else 0.1, ratio=ratio)
+    out = {}
+    try:
+        for lanes in (1, 2, 4, 8):
+            capi.set_team_lanes(lanes)
+            e = gpu.Engine(x, D, box=box, max_steps=300, neighbor_mode=4)
+            e.run(0)
+            out[lanes] = e.result()
+    finally:
+        capi.set_team_lanes(0)
+    ref = out[1]
+    for lanes in (2, 4, 8):
+        r = out[lanes]
+        assert r["steps"] == ref["steps"] == 300, lanes
+        for key in ("positions", "diameters", "particle_origin", "mu_flag_history") + HIST:
+            assert np.array_equal(np.asarray(r[key]), np.asarray(ref[key])), (lanes, key)
+        assert r["phi"] == ref["phi"] and r["max_min_dist"] == ref["max_min_dist"], lanes
+
+
+def test_team_lanes_batched_engine(gpu):
+    """The batched engine's step with per-thread and 8-lane teams: identical
+    results for every packing."""
+    import rcpgenerator_amd as rcp
+    from rcpgenerator_amd import capi
+    cases = _batch_cases() + _wide_cases()
+    opts = {"max_steps": 600}
+    out = {}
+    try:
+        for lanes in (1, 8):
+            capi.set_team_lanes(lanes)
+            out[lanes] = rcp.run_packing_batch([c[0] for c in cases], [c[1] for c in cases], opts)
+    finally:
+        capi.set_team_lanes(0)
+    for a, b in zip(out[1], out[8]):
+        _assert_same_run(a, b)
+    with pytest.raises(capi.RcpError):
+        capi.set_team_lanes(3)
+
+
 def test_batch_pool_matches_single_runs(gpu):
     """pool=True runs each packing on its own engine, as run_packing does:
     every result is identical to the single run's."""
