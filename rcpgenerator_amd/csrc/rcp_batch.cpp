@@ -65,6 +65,8 @@ struct Member {
     size_t row0 = 0;                   // first partial row
     int nb = 0;                        // workgroups of the per-thread step
     int wide_blocks = 0;               // workgroups of the one-wave step
+    size_t woff = 0;                   // first contact list of its one-wave particles
+    int wn = 0;                        // contact lists (bound on one-wave particles)
     int cx = 0, cm = 0, cd = 0, co = 0;   // current buffer of x, (m, v), (D0, tag), origin
     MultiGrid mg;
     Engine::StepPlan plan;
@@ -105,6 +107,11 @@ private:
     uint32_t *inv_tag_ = nullptr, *tag_tmp_ = nullptr, *cell_of_ = nullptr, *slot_ = nullptr,
              *perm_ = nullptr, *cell_count_ = nullptr, *cell_start_ = nullptr,
              *big_ = nullptr, *big_count_ = nullptr;
+    // contact lists of the members' one-wave particles (zeroed at upload;
+    // the one-wave step resets the counters it consumes)
+    uint32_t* wide_count_ = nullptr;
+    unsigned long long* wide_keys_ = nullptr;
+    size_t wide_total_ = 0;
     unsigned long long* origin_[2] = {};
     double* partials_ = nullptr;
     double* results_ = nullptr;
@@ -137,10 +144,13 @@ private:
 
 void Group::allocate()
 {
-    size_t NT = 0, CT = 0, rows = 0, maxN = 0;
+    size_t NT = 0, CT = 0, rows = 0, maxN = 0, WT = 0;
     for (Member& M : m_) {
         M.poff = NT;
         M.coff = CT;
+        M.woff = WT;
+        M.wn = (int)std::min<long long>(M.ctl->wide_bound(), (long long)M.N);
+        WT += (size_t)M.wn;
         M.cells = M.ctl->cell_budget();
         NT += (size_t)M.N;
         CT += (size_t)M.cells + 1;
@@ -176,6 +186,9 @@ void Group::allocate()
     const size_t occ = take(CT * sizeof(uint32_t)), ocs = take(CT * sizeof(uint32_t));
     const size_t obig = take(2 * (NT / 16 + m_.size() + 1) * sizeof(uint32_t)),
                  obc = take(sizeof(uint32_t));
+    const size_t owc = take(WT * sizeof(uint32_t)),
+                 owk = take(WT * kWideListCap * sizeof(unsigned long long));
+    wide_total_ = WT;
     const size_t opart = take(rows * S_NUM * sizeof(double));
     const size_t ores = take(m_.size() * S_NUM * sizeof(double));
     const size_t otx = take(maxN * d * sizeof(double)), otx2 = take(maxN * sizeof(double));
@@ -204,6 +217,8 @@ void Group::allocate()
     cell_start_ = reinterpret_cast<uint32_t*>(b + ocs);
     big_ = reinterpret_cast<uint32_t*>(b + obig);
     big_count_ = reinterpret_cast<uint32_t*>(b + obc);
+    wide_count_ = reinterpret_cast<uint32_t*>(b + owc);
+    wide_keys_ = reinterpret_cast<unsigned long long*>(b + owk);
     partials_ = reinterpret_cast<double*>(b + opart);
     results_ = reinterpret_cast<double*>(b + ores);
     tmp_ = reinterpret_cast<double*>(b + otx);
@@ -255,6 +270,9 @@ void Group::upload()
     hip_check(hipMemcpyAsync(origin_[0], org.data(), org.size() * sizeof(unsigned long long),
                              hipMemcpyHostToDevice, s_), "upload origin");
     hip_check(hipMemsetAsync(m_arr_[0], 0, NT * d * sizeof(double), s_), "zero m");
+    if (wide_total_)
+        hip_check(hipMemsetAsync(wide_count_, 0, wide_total_ * sizeof(uint32_t), s_),
+                  "zero contact lists");
     hip_check(hipMemsetAsync(v_[0], 0, NT * d * sizeof(double), s_), "zero v");
     hip_check(hipStreamSynchronize(s_), "upload sync");
     live_.resize(m_.size());
@@ -382,6 +400,12 @@ void Group::stage_and_launch(bool do_adam)
         a.partials = partials_ + M.row0 * S_NUM;
         a.tile_len = M.nb;
         a.pad = l.pad;
+        if (M.wn > 0 && M.mg.wide > 0) {
+            a.wide_count = wide_count_ + M.woff;
+            a.wide_keys = wide_keys_ + M.woff * kWideListCap;
+            a.wide_list_n = M.wn;
+            a.wide_cap = kWideListCap;
+        }
         P.mg = M.mg;
         P.mg.grow = l.grow;
         P.wide_blocks = M.mg.wide > 0 ? std::max(1, M.wide_blocks) : 0;

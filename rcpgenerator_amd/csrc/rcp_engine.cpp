@@ -327,6 +327,19 @@ void DeviceState::ensure_inv_tag()
     inv_tag_valid = true;
 }
 
+void DeviceState::ensure_wide_lists(int n)
+{
+    if (n <= wide_list_n) return;
+    if (wide_count) {
+        release(wide_count);
+        release(wide_keys);
+    }
+    wide_count = alloc<uint32_t>((size_t)n);
+    wide_keys = alloc<unsigned long long>((size_t)n * kWideListCap);
+    hip_check(hipMemsetAsync(wide_count, 0, (size_t)n * sizeof(uint32_t), s), "wide list clear");
+    wide_list_n = n;
+}
+
 void DeviceState::ensure_partials(size_t blocks)
 {
     if (blocks <= partial_blocks) return;
@@ -967,6 +980,13 @@ Engine::Scalars Engine::step_device(bool do_adam, double alpha, double inv_b1, d
         S.ensure_partials(multi_partial_rows(N_) + 1);
         a.pad = 0.5 * tile_skin_;
         mg_.grow = std::max(1.0, kappa_D_ / kappa_b_);
+        if (mg_.wide > 0 && wide_bound_ > 0) {
+            S.ensure_wide_lists((int)std::min<long long>(wide_bound_, (long long)N_));
+            a.wide_count = S.wide_count;
+            a.wide_keys = S.wide_keys;
+            a.wide_list_n = S.wide_list_n;
+            a.wide_cap = kWideListCap;
+        }
     }
     if (opt_.exact_reductions) S.ensure_inv_tag();
     a.partials = S.partials;

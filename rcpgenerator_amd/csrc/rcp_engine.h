@@ -100,6 +100,12 @@ public:
     uint32_t *cell_of = nullptr, *slot = nullptr, *perm = nullptr, *cell_count = nullptr,
              *cell_start = nullptr, *row_count = nullptr, *row_start = nullptr, *nbr = nullptr;
     uint32_t *big = nullptr, *big_count = nullptr;   // crowded cells of a binning
+    // contact lists of the one-wavefront particles (size classes), zeroed
+    // once: the one-wavefront step resets the counters it consumes
+    uint32_t* wide_count = nullptr;
+    unsigned long long* wide_keys = nullptr;
+    int wide_list_n = 0;
+    void ensure_wide_lists(int n);
     size_t cell_cap = 0, nbr_cap = 0;
     // exact reductions
     double *contrib = nullptr, *mag = nullptr, *ref_terms = nullptr;

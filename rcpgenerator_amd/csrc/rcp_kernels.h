@@ -103,7 +103,19 @@ struct StepArgs {
     // +-tile_span cells along every axis holding more than one cell
     int tile_len = 0, tiles_per_row = 0, hc_cap = 0, h_cap = 0, list_cap = 0, tile_span = 1;
     double dmax = 0.0, pad = 0.0;
+    // size classes: contacts of the one-wavefront particles (working
+    // positions below wide_list_n) with smaller particles, found by the
+    // per-thread step: wide_count[w] keys (tag << 32 | position) at
+    // wide_keys[w * wide_cap ..]; the one-wavefront step reads them and
+    // resets wide_count[w] to 0 (null: no lists)
+    uint32_t* wide_count = nullptr;
+    unsigned long long* wide_keys = nullptr;
+    int wide_list_n = 0, wide_cap = 0;
 };
+
+// Keys per one-wavefront particle in the contact lists of the size-class
+// step (more contacts: that particle scans its stencils itself).
+constexpr int kWideListCap = 256;
 
 // Tiled step over the binned working order (ndim 2 and 3).  One workgroup
 // owns tile_len consecutive cells of one row along axis 0; it stages the
@@ -133,11 +145,12 @@ size_t step_partial_blocks(size_t n);
 // given it is recorded between the two (timing of the step kernel alone).
 // Lanes per particle of the size-class step (per-thread kernel or teams of
 // 2, 4 or 8 lanes; 4 at most from five dimensions up): the smallest team
-// that gives the launch kTeamThreads threads (about eight waves on every
-// SIMD of the chip).  Results do not depend on it.  set_team_lanes(n > 0)
+// that gives the launch kTeamThreads threads.  The figure is measured on
+// batched 4-D launches, where 22k, 56k and 141k particles ran fastest with
+// 8, 4 and 2 lanes.  Results do not depend on it.  set_team_lanes(n > 0)
 // forces n lanes (rounded down to a power of two within those limits; for
 // measurements), 0 restores the rule.
-constexpr size_t kTeamThreads = size_t(1) << 18;
+constexpr size_t kTeamThreads = 160000;
 int team_lanes_log2(size_t particles, int ndim);
 void set_team_lanes(int lanes);
 int launch_step_list(const StepArgs& a, hipStream_t s, hipEvent_t mark = nullptr);

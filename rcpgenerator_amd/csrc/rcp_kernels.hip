@@ -1090,6 +1090,19 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Contact of the per-thread (or team) step's particle q (tag tq) with a
+// one-wavefront particle w: appended to w's list for the one-wavefront step
+// (the overlap predicate is symmetric bit for bit, so w's list holds exactly
+// its contacts with smaller classes).
+__device__ __forceinline__ void note_wide_contact(const StepArgs& a, uint32_t w, uint32_t tq,
+                                                  uint32_t q)
+{
+    if (w >= (uint32_t)a.wide_list_n) return;
+    const uint32_t slot = atomicAdd(&a.wide_count[w], 1u);
+    if (slot < (uint32_t)a.wide_cap)
+        a.wide_keys[(size_t)w * a.wide_cap + slot] = ((unsigned long long)tq << 32) | q;
+}
+
 // More contacts than a buffer holds: the contacts of particle q in tag
 // order by repeated scans of its stencils (each scan takes the next tag).
 template <int ND>
@@ -1135,6 +1148,7 @@ __device__ __forceinline__ void multi_particle(const StepArgs& a, const MultiGri
         for_each_class_candidate<ND>(a, mg, o.x, o.D, [&](uint32_t qj) {
             if (qj == (uint32_t)q) return;
             if (!overlaps<ND>(a, o, qj)) return;
+            if (qj < n_wide && a.wide_keys) note_wide_contact(a, qj, o.tag, (uint32_t)q);
             if (n < kMultiCap) buf[n][tid] = qj;
             ++n;
         });
@@ -1217,6 +1231,7 @@ __device__ __forceinline__ void team_particle(const StepArgs& a, const MultiGrid
         for_each_class_cell<ND>(a, mg, o.x, o.D, tl, lanes, [&](uint32_t q0, uint32_t q1) {
             for (uint32_t qj = q0; qj < q1; ++qj) {
                 if (qj == (uint32_t)q || !overlaps<ND>(a, o, qj)) continue;
+                if (qj < n_wide && a.wide_keys) note_wide_contact(a, qj, o.tag, (uint32_t)q);
                 const int slot = atomicAdd(&T.cnt[team], 1);
                 if (slot < kMultiCap)
                     T.keys[team][slot] = ((unsigned long long)a.tag[qj] << 32) | qj;
@@ -1318,6 +1333,7 @@ constexpr int kWideWaves = 4;                  // wavefronts per workgroup
 constexpr int kWideGrid = 1024;                // persistent workgroups
 constexpr int kWideCap = 1024;                 // contacts buffered per particle
 constexpr int kWideBlock = kWideWaves * kWave;
+static_assert(kWideListCap <= kWideCap, "a listed particle's keys fit its LDS buffer");
 
 template <int ND>
 struct WideLds {
@@ -1346,10 +1362,22 @@ __device__ __forceinline__ void wide_particles(const StepArgs& a, const MultiGri
     for (uint32_t q = blk * kWideWaves + wv; q < n_wide; q += waves) {
         Own<ND> o;
         load_own<ND>(a, (int)q, o);
-        if (lane == 0) cnt[wv] = 0;
+        // contacts with the smaller classes, listed by the per-thread step
+        // (unless the list overflowed); the counter is reset for the next step
+        const bool has_list = a.wide_keys && q < (uint32_t)a.wide_list_n;   // wave-uniform
+        uint32_t nl = 0u;
+        if (has_list && lane == 0) nl = atomicExch(&a.wide_count[q], 0u);
+        nl = (uint32_t)readlane_i32((int)nl, 0);
+        const bool listed = has_list && nl <= (uint32_t)a.wide_cap;
+        if (listed)
+            for (uint32_t i = lane; i < nl; i += kWave)
+                keys[wv][i] = a.wide_keys[(size_t)q * a.wide_cap + i];
+        if (lane == 0) cnt[wv] = listed ? (int)nl : 0;
         wave_sync();
         // ---- collect: lanes take the stencil cells of every class in turn
-        for (int b = 0; b < mg.C; ++b) {
+        // (only the one-wavefront classes when the list holds the rest)
+        const int nclass = listed ? mg.wide : mg.C;
+        for (int b = 0; b < nclass; ++b) {
             const CellGrid& gr = mg.grid[b];
             const double R = 0.5 * (o.D + 2.0 * mg.thr[b] * mg.grow) + a.pad;
             int lo[ND], ct[ND];
