@@ -38,6 +38,17 @@ def ref(oracle):
     return oracle
 
 
+def load_build_recipe():
+    """The package's build recipe, loaded by path: importing the package
+    fails until its native module exists."""
+    import importlib.util
+    path = os.path.join(ROOT, "rcpgenerator_amd", "_build.py")
+    spec = importlib.util.spec_from_file_location("rcpgenerator_amd_build", path)
+    recipe = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(recipe)
+    return recipe
+
+
 def load_golden(name):
     with np.load(os.path.join(GOLDEN, name), allow_pickle=False) as z:
         return {k: z[k] for k in z.files}

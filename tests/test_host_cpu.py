@@ -11,12 +11,12 @@ import re
 import numpy as np
 import pytest
 
-from conftest import ROOT, load_golden
+from conftest import ROOT, load_build_recipe, load_golden
 
 
 @pytest.fixture(scope="session")
 def native():
-    from rcpgenerator_amd import _build
+    _build = load_build_recipe()
     if not _build.built():
         _build.build()
     import rcpgenerator_amd
