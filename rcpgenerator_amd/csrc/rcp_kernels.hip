@@ -1717,6 +1717,30 @@ __global__ __launch_bounds__(kWave) void k_exact_reduce(StepArgs a, const uint32
 
 // ---------------------------------------------------------------- binning
 
+// Slot of this lane's particle in `cell`.  Lanes whose cell repeats their
+// left neighbour's form a run, and the run's first lane reserves the run's
+// slots with one atomic: the working order keeps a cell's members together,
+// so a wave issues a few atomics instead of one per particle.  Slots only
+// need to be distinct (the cells are sorted by working position
+// afterwards).  The active lanes of the calling wave must be a prefix of it
+// (w < N with w increasing with the lane), and every one of them calls.
+__device__ __forceinline__ uint32_t run_slot(uint32_t* count, uint32_t cell)
+{
+    const int lane = threadIdx.x % kWave;
+    const uint32_t left = __shfl_up(cell, 1, kWave);
+    const bool head = lane == 0 || left != cell;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long active = __ballot(true);
+    const unsigned long long upto = (2ull << lane) - 1ull;   // lanes 0..lane
+    const unsigned long long above = heads & ~upto;
+    const int end = above ? __ffsll((long long)above) - 1 : __popcll(active);
+    uint32_t base = 0u;
+    if (head) base = atomicAdd(&count[cell], (uint32_t)(end - lane));
+    const int h = 63 - __clzll((long long)(heads & upto));
+    base = __shfl(base, h, kWave);
+    return base + (uint32_t)(lane - h);
+}
+
 template <int ND>
 __device__ __forceinline__ void bin_one(const BinArgs& a, int w)
 {
@@ -1729,7 +1753,7 @@ __device__ __forceinline__ void bin_one(const BinArgs& a, int w)
         stride *= a.grid.nc[d];
     }
     a.cell_of[w] = (uint32_t)c;
-    a.slot[w] = atomicAdd(&a.cell_count[c], 1u);
+    a.slot[w] = run_slot(a.cell_count, (uint32_t)c);
 }
 
 template <int ND>
@@ -1756,7 +1780,7 @@ __device__ __forceinline__ void bin_multi_one(const BinArgs& a, const MultiGrid&
     }
     cell += mg.base[c];
     a.cell_of[w] = (uint32_t)cell;
-    a.slot[w] = atomicAdd(&a.cell_count[cell], 1u);
+    a.slot[w] = run_slot(a.cell_count, (uint32_t)cell);
 }
 
 template <int ND>

@@ -193,12 +193,17 @@ def _initialize(cases, threads: int):
         return list(ex.map(capi.initialize, cases))
 
 
-def _batch(inits, options, threads: int = 8):
+def _batch(inits, options, threads: int = 8, pool: bool = False):
     import time
     runs = [(x, D, b, w, o) for (x, D, b, w), o in zip(inits, options)]
     t0 = time.perf_counter()
+    def finished(done, total):
+        if done % 16 == 0 or done == total:
+            print(f"[surrogate] {done}/{total} finished, {time.perf_counter() - t0:.0f}s",
+                  flush=True)
+
     res, st = capi.run_batch(
-        runs, threads=threads, every=2000,
+        runs, threads=threads, pool=pool, every=2000, progress=finished if pool else None,
         rounds_progress=lambda r, f, n: print(f"[surrogate] round {r}: {f}/{n} finished, "
                                               f"{time.perf_counter() - t0:.0f}s", flush=True))
     wall = time.perf_counter() - t0
@@ -255,13 +260,17 @@ def choose_schedule(pred_row) -> int:
     return -1
 
 
-def evaluate(surr: "Surrogate", cases, threads: int = 8, log=print, detail: bool = False) -> dict:
+def evaluate(surr: "Surrogate", cases, threads: int = 8, log=print, detail: bool = False,
+             engine_legs: bool = False) -> dict:
     """Plain batched runs against surrogate-guided ones on the same packings.
     Guided: each packing takes the fastest schedule the network calls safe
     (or the default one); a shortened run whose phi misses the predicted
     phi_J by more than RERUN_TOL is run again with the default schedule, and
     its cost counts both runs.  Also reported: every packing on the safest
-    shortened schedule, the method without the network."""
+    shortened schedule, the method without the network.  With engine_legs,
+    the plain method also runs on the single-packing engine: one packing at
+    a time (a loop over Packing.pack()) and as a pool of `threads` engines
+    running side by side on the device."""
     inits = _initialize(cases, threads)
     plain, wall_p, _ = _batch(inits, [None] * len(inits), threads)
     X = np.stack([features_of_packing(D, c["Ndim"], list(w)) for c, (x, D, b, w) in
@@ -312,6 +321,20 @@ def evaluate(surr: "Surrogate", cases, threads: int = 8, log=print, detail: bool
                            "phi_mean_diff": float(np.mean(phi_f - phi_p)),
                            "jammed": float(np.mean([r["max_min_dist"] < 5e-4 for r in fixed]))},
     }
+    if engine_legs:
+        # the plain GPU engine without batching: packings one after another,
+        # then `threads` single engines at a time
+        serial, wall_s, _ = _batch(inits, [None] * len(inits), 1, pool=True)
+        pooled, wall_e, _ = _batch(inits, [None] * len(inits), threads, pool=True)
+        for name, res, wall in (("engine_serial", serial, wall_s),
+                                ("engine_pool", pooled, wall_e)):
+            phi_e = np.array([r["phi"] for r in res])
+            out[name] = {"wall_s": wall, "threads": 1 if name == "engine_serial" else threads,
+                         "steps": int(sum(r["steps"] for r in res)),
+                         "jammed": float(np.mean([r["max_min_dist"] < 5e-4 for r in res])),
+                         "phi_mean": float(phi_e.mean()),
+                         "guided_speedup_wall": wall / (wall_g + wall_r),
+                         "guided_phi_mean_diff": float(np.mean(phi_g) - phi_e.mean())}
     log(f"[surrogate] evaluation: {out}")
     if detail:
         out.update({"choice": choice, "rerun_ids": miss,
