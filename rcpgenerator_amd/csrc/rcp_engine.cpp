@@ -179,7 +179,9 @@ DeviceState::DeviceState(int N_, int nd_, bool with_fixed, bool exact_, hipStrea
             ref_start = alloc<uint32_t>((size_t)N + 1);
         }
         hip_check(hipHostMalloc((void**)&h_result, S_NUM * sizeof(double),
-                                hipHostMallocDefault), "hipHostMalloc");
+                                hipHostMallocMapped | hipHostMallocCoherent), "hipHostMalloc");
+        hip_check(hipHostGetDevicePointer((void**)&h_result_dev, h_result, 0),
+                  "hipHostGetDevicePointer");
         ensure_scan(std::max(sort_hist_entries(N), (size_t)N) + 8);
     } catch (...) {
         for (void* p : allocs_) dev_free(p);
@@ -970,7 +972,7 @@ Engine::Scalars Engine::step_device(bool do_adam, double alpha, double inv_b1, d
     a.do_adam = do_adam ? 1 : 0;
     a.inv_tag = S.inv_tag;
     a.partials = S.partials;
-    a.result = S.result;
+    a.result = S.h_result_dev;
     if (mode == Mode::Tile) {
         S.ensure_partials((size_t)tile_plan_.blocks + 1);
         a.dmax = dmax_now();
@@ -1009,8 +1011,7 @@ Engine::Scalars Engine::step_device(bool do_adam, double alpha, double inv_b1, d
         if (launch_step_list(a, stream_, mark)) throw HipFailure("list step launch");
     }
     if (opt_.time_kernels) timer_pending_[1] = true;   // mark was recorded after the kernel
-    hip_check(hipMemcpyAsync(S.h_result, S.result, S_NUM * sizeof(double), hipMemcpyDeviceToHost,
-                             stream_), "scalars download");
+    // the fold wrote the scalars to h_result; completion makes them visible
     hip_check(hipStreamSynchronize(stream_), "step sync");
     flush_timers();
     Scalars s;

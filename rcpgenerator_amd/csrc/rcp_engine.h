@@ -110,8 +110,11 @@ public:
     // exact reductions
     double *contrib = nullptr, *mag = nullptr, *ref_terms = nullptr;
     uint32_t *ref_count = nullptr, *ref_start = nullptr;
-    // reductions
-    double *partials = nullptr, *result = nullptr, *h_result = nullptr;
+    // reductions; the step's scalars are written by the fold kernel straight
+    // into coherent pinned host memory (h_result, device address
+    // h_result_dev), so no copy follows the step
+    double *partials = nullptr, *result = nullptr, *h_result = nullptr,
+           *h_result_dev = nullptr;
     // sorting
     SortTmp sort;
     char* scan_tmp = nullptr;

@@ -658,15 +658,21 @@ __device__ __forceinline__ void tile_particles_lds(
         double F[ND];
 #pragma unroll
         for (int d = 0; d < ND; ++d) F[d] = 0.0;
+        // The staged sD holds half diameters, so r = hD + sD[j] is the
+        // reference's 0.5 * (Di + Dj) bit for bit (halving is exact and
+        // commutes with rounding), and d2 starts from the first square
+        // (0.0 + dd * dd is dd * dd: a square is never -0.0).
+        const double hD = 0.5 * o.D;
         auto hit = [&](uint32_t j) {
-            double d2 = 0.0;
+            double d2;
 #pragma unroll
             for (int d = 0; d < ND; ++d) {
                 double dd = sx[d * h_cap + j] - o.x[d];
                 if (!PLAIN) dd = mic(dd, a.g.inv_box[d], a.g.box[d]);
-                d2 += dd * dd;
+                if (d == 0) d2 = dd * dd;
+                else d2 += dd * dd;
             }
-            const double r = 0.5 * (o.D + sD[j]);
+            const double r = hD + sD[j];
             return d2 < r * r;
         };
         int n = 0;
@@ -725,7 +731,8 @@ __device__ __forceinline__ void tile_particles_lds(
                 const uint32_t j = mylist[p * BLOCK];
                 double xj[ND];
                 pos(j, xj);
-                pair_contact<ND>(a, o, xj, sD[j], rk ? srk[j] : 0.0, stag[j], F, acc, nullptr);
+                pair_contact<ND>(a, o, xj, 2.0 * sD[j], rk ? srk[j] : 0.0, stag[j], F, acc,
+                                 nullptr);
             }
         } else {
             // more contacts than the buffer: take them in tag order by
@@ -744,15 +751,15 @@ __device__ __forceinline__ void tile_particles_lds(
                         if (key >= best) continue;
                         double xj[ND];
                         pos(j, xj);
-                        if (overlap_at<ND>(a, o, xj, sD[j])) best = key;
+                        if (overlap_at<ND>(a, o, xj, 2.0 * sD[j])) best = key;
                     }
                 }
                 if (best == ~0ull) break;
                 const uint32_t j = (uint32_t)(best & 0xffffffffull);
                 double xj[ND];
                 pos(j, xj);
-                pair_contact<ND>(a, o, xj, sD[j], rk ? srk[j] : 0.0, (uint32_t)(best >> 32), F,
-                                 acc, nullptr);
+                pair_contact<ND>(a, o, xj, 2.0 * sD[j], rk ? srk[j] : 0.0,
+                                 (uint32_t)(best >> 32), F, acc, nullptr);
                 last = (long long)(best >> 32);
             }
         }
@@ -940,7 +947,7 @@ __global__ __launch_bounds__(BLOCK) void k_step_tile(StepArgs a, long long ntile
                 const bool fp = a.fixed && a.fixed[p];
                 const double D0p = a.D0[p];
                 const double Dp = fp ? D0p : D0p * a.kappa;
-                sD[i] = Dp;
+                sD[i] = 0.5 * Dp;   // half diameter (exact)
                 if (rk_lds) srk[i] = a.rk ? a.rk[p] : (fp ? 0.0 : Dp * 0.5);
                 stag[i] = a.tag[p];
             }
