@@ -12,6 +12,7 @@
 #include "rcp_tile_table.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -178,10 +179,13 @@ DeviceState::DeviceState(int N_, int nd_, bool with_fixed, bool exact_, hipStrea
             ref_count = alloc<uint32_t>((size_t)N + 1);
             ref_start = alloc<uint32_t>((size_t)N + 1);
         }
-        hip_check(hipHostMalloc((void**)&h_result, S_NUM * sizeof(double),
+        hip_check(hipHostMalloc((void**)&h_result, (S_NUM + 1) * sizeof(double),
                                 hipHostMallocMapped | hipHostMallocCoherent), "hipHostMalloc");
         hip_check(hipHostGetDevicePointer((void**)&h_result_dev, h_result, 0),
                   "hipHostGetDevicePointer");
+        h_flag = reinterpret_cast<unsigned long long*>(h_result + S_NUM);
+        h_flag_dev = reinterpret_cast<unsigned long long*>(h_result_dev + S_NUM);
+        *h_flag = 0;
         ensure_scan(std::max(sort_hist_entries(N), (size_t)N) + 8);
     } catch (...) {
         for (void* p : allocs_) dev_free(p);
@@ -198,6 +202,28 @@ DeviceState::~DeviceState()
 }
 
 void DeviceState::sync() { hip_check(hipStreamSynchronize(s), "sync"); }
+
+// Polls the completion flag instead of a blocking wait on the stream (whose
+// wake-up costs about as much as a small packing's whole step).  The stream
+// is queried now and then, so a failed launch is reported, and a stream
+// that finished without the flag is an error rather than a hang.
+void DeviceState::wait_flag(unsigned long long seq)
+{
+    const volatile unsigned long long* f = h_flag;
+    for (unsigned spin = 1;; ++spin) {
+        if (*f == seq) break;
+        if ((spin & 1023u) == 0) {
+            const hipError_t e = hipStreamQuery(s);
+            if (e == hipSuccess) {
+                if (*f == seq) break;
+                throw HipFailure("step finished without its completion flag");
+            }
+            if (e != hipErrorNotReady) hip_check(e, "step");
+        }
+        __builtin_ia32_pause();
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+}
 
 void DeviceState::upload(const double* xh, const double* D0h, const int8_t* fixedh)
 {
@@ -746,6 +772,8 @@ void Engine::flush_timers()
     for (int slot = 0; slot < 2; ++slot) {
         if (!timer_pending_[slot]) continue;
         float ms = 0.0f;
+        // a polled step may return before the stream has retired the event
+        hip_check(hipEventSynchronize(ev_[2 * slot + 1]), "event sync");
         hip_check(hipEventElapsedTime(&ms, ev_[2 * slot], ev_[2 * slot + 1]), "event time");
         if (slot == 0) {
             t_build_ms_ += ms;
@@ -992,6 +1020,12 @@ Engine::Scalars Engine::step_device(bool do_adam, double alpha, double inv_b1, d
     }
     if (opt_.exact_reductions) S.ensure_inv_tag();
     a.partials = S.partials;
+    // the exact path's fold is not its last kernel: it waits on the stream
+    const bool poll = !opt_.exact_reductions;
+    if (poll) {
+        a.done_flag = S.h_flag_dev;
+        a.done_seq = ++S.flag_seq;
+    }
     begin_timer(1);
     hipEvent_t mark = opt_.time_kernels ? ev_[3] : nullptr;
     if (mode == Mode::Tile) {
@@ -1011,8 +1045,9 @@ Engine::Scalars Engine::step_device(bool do_adam, double alpha, double inv_b1, d
         if (launch_step_list(a, stream_, mark)) throw HipFailure("list step launch");
     }
     if (opt_.time_kernels) timer_pending_[1] = true;   // mark was recorded after the kernel
-    // the fold wrote the scalars to h_result; completion makes them visible
-    hip_check(hipStreamSynchronize(stream_), "step sync");
+    // the fold wrote the scalars to h_result
+    if (poll) S.wait_flag(a.done_seq);
+    else hip_check(hipStreamSynchronize(stream_), "step sync");
     flush_timers();
     Scalars s;
     for (int k = 0; k < S_NUM; ++k) s.v[k] = S.h_result[k];

@@ -112,9 +112,14 @@ public:
     uint32_t *ref_count = nullptr, *ref_start = nullptr;
     // reductions; the step's scalars are written by the fold kernel straight
     // into coherent pinned host memory (h_result, device address
-    // h_result_dev), so no copy follows the step
+    // h_result_dev), so no copy follows the step.  The word after them is the
+    // completion flag: the fold stores the step's sequence number there
+    // last, and wait_flag polls it.
     double *partials = nullptr, *result = nullptr, *h_result = nullptr,
            *h_result_dev = nullptr;
+    unsigned long long *h_flag = nullptr, *h_flag_dev = nullptr;
+    unsigned long long flag_seq = 0;
+    void wait_flag(unsigned long long seq);
     // sorting
     SortTmp sort;
     char* scan_tmp = nullptr;

@@ -96,6 +96,11 @@ struct StepArgs {
     const uint32_t* inv_tag = nullptr; // N: working position of reference index
     double* partials = nullptr;      // gridDim * S_NUM rows, folded into result
     double* result = nullptr;        // S_NUM totals
+    // optional: the step's fold (its last kernel) stores done_seq here at
+    // system scope once result is written (tile, list, size-class and
+    // direct steps)
+    unsigned long long* done_flag = nullptr;
+    unsigned long long done_seq = 0;
     // tiled step (launch_step_tile): plan from plan_tiles; dmax = largest
     // current diameter, pad = largest drift of a particle from its binned
     // cell (half the skin)

@@ -139,7 +139,19 @@ __device__ __forceinline__ void finish_grid(const StepArgs& a, double (&blk)[S_N
 constexpr int kFoldBlock = 1024;
 
 // Rows b = t, t + kFoldBlock, ... go to thread t; then one block reduction.
-__device__ __forceinline__ void fold_rows(const double* partials, unsigned rows, double* result)
+// With `flag`, thread 0 then publishes `seq` there at system scope, after
+// the totals (the host may poll it instead of waiting for the stream).
+__device__ __forceinline__ void publish(unsigned long long* flag, unsigned long long seq)
+{
+    if (flag) {
+        __threadfence_system();
+        *reinterpret_cast<volatile unsigned long long*>(flag) = seq;
+    }
+}
+
+__device__ __forceinline__ void fold_rows(const double* partials, unsigned rows, double* result,
+                                          unsigned long long* flag = nullptr,
+                                          unsigned long long seq = 0)
 {
     double v[S_NUM];
 #pragma unroll
@@ -158,13 +170,53 @@ __device__ __forceinline__ void fold_rows(const double* partials, unsigned rows,
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < S_NUM; ++k) result[k] = tot[k];
+        publish(flag, seq);
     }
 }
 
 __global__ __launch_bounds__(kFoldBlock) void k_fold(const double* partials, unsigned rows,
-                                                     double* result)
+                                                     double* result, unsigned long long* flag,
+                                                     unsigned long long seq)
 {
-    fold_rows(partials, rows, result);
+    fold_rows(partials, rows, result, flag, seq);
+}
+
+// k_fold for at most kWave rows, in one wavefront and with k_fold's bits:
+// there every row sits in wave 0, the other waves reduce to +0.0, and
+// adding +0.0 (or taking the larger of +0.0 and the value) changes the
+// wave's total only where it is -0.0 (or below zero), as done here once.
+__global__ __launch_bounds__(kWave) void k_fold_wave(const double* partials, unsigned rows,
+                                                     double* result, unsigned long long* flag,
+                                                     unsigned long long seq)
+{
+    const unsigned t = threadIdx.x;
+    double v[S_NUM];
+#pragma unroll
+    for (int k = 0; k < S_NUM; ++k) {
+        v[k] = 0.0;
+        if (t < rows) {
+            const double x = partials[(size_t)t * S_NUM + k];
+            if (is_max_slot(k)) v[k] = (x > v[k]) ? x : v[k];
+            else v[k] += x;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < S_NUM; ++k) {
+        const double tot = is_max_slot(k) ? wave_max(v[k]) : wave_sum(v[k]);
+        if (t == 0) result[k] = is_max_slot(k) ? ((0.0 > tot) ? 0.0 : tot) : tot + 0.0;
+    }
+    if (t == 0) publish(flag, seq);
+}
+
+// The fold of `rows` partial rows into result[S_NUM] (and the completion
+// flag, when given).
+inline void launch_fold(const double* partials, unsigned rows, double* result, hipStream_t s,
+                        unsigned long long* flag = nullptr, unsigned long long seq = 0)
+{
+    if (rows <= (unsigned)kWave)
+        k_fold_wave<<<1, kWave, 0, s>>>(partials, rows, result, flag, seq);
+    else
+        k_fold<<<1, kFoldBlock, 0, s>>>(partials, rows, result, flag, seq);
 }
 
 // ------------------------------------------------------ per-particle parts
@@ -397,15 +449,35 @@ __global__ __launch_bounds__(kBlock) void k_step_list(StepArgs a, uint32_t* vcou
         for (int d = 0; d < ND; ++d) F[d] = 0.0;
         const uint32_t e0 = a.row_start[q], e1 = a.row_start[q + 1];
         uint32_t nvalid = 0;
-        for (uint32_t e = e0; e < e1; ++e) {
-            const uint32_t qj = a.nbr[e];
-            const uint32_t tj = a.tag[qj];
-            if (a.contrib) {
-                double* rec = a.contrib + (size_t)e * 4;
-                if (contact<ND>(a, o, qj, tj, F, acc, rec)) ++nvalid;
-                else rec[1] = 0.0;
-            } else {
-                contact<ND>(a, o, qj, tj, F, acc, nullptr);
+        // The row is taken in chunks: every load of a chunk is issued before
+        // its first pair is added, so a row costs one memory round trip per
+        // chunk instead of one per entry; the pairs are still added in row
+        // (tag) order.
+        constexpr uint32_t kChunk = ND <= 3 ? 8u : 4u;
+        for (uint32_t e = e0; e < e1; e += kChunk) {
+            const uint32_t n = (e1 - e < kChunk) ? e1 - e : kChunk;
+            uint32_t qj[kChunk], tj[kChunk];
+            double xj[kChunk][ND], Dj[kChunk], rkj[kChunk];
+#pragma unroll
+            for (uint32_t u = 0; u < kChunk; ++u)
+                qj[u] = (u < n) ? a.nbr[e + u] : (uint32_t)q;
+#pragma unroll
+            for (uint32_t u = 0; u < kChunk; ++u) {
+                load_partner<ND>(a, qj[u], xj[u], Dj[u], rkj[u]);
+                tj[u] = a.tag[qj[u]];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kChunk; ++u) {
+                if (u >= n) break;
+                if (a.contrib) {
+                    double* rec = a.contrib + (size_t)(e + u) * 4;
+                    if (pair_contact<ND>(a, o, xj[u], Dj[u], rkj[u], tj[u], F, acc, rec))
+                        ++nvalid;
+                    else
+                        rec[1] = 0.0;
+                } else {
+                    pair_contact<ND>(a, o, xj[u], Dj[u], rkj[u], tj[u], F, acc, nullptr);
+                }
             }
         }
         if (vcount) vcount[o.tag] = nvalid;
@@ -2736,7 +2808,7 @@ int launch_step_tile(const StepArgs& a0, const TilePlan& p, hipStream_t s, hipEv
         default: return 2;
     }
     if (mark && hipEventRecord(mark, s) != hipSuccess) return 1;
-    k_fold<<<1, kFoldBlock, 0, s>>>(a.partials, g, a.result);
+    launch_fold(a.partials, g, a.result, s, a.done_flag, a.done_seq);
     return check_launch();
 }
 
@@ -2746,7 +2818,7 @@ int launch_step_list(const StepArgs& a, hipStream_t s, hipEvent_t mark)
     const unsigned g = grid_for(a.g.N);
     RCP_DISPATCH_ND(a.g.ndim, k_step_list, <<<g, kBlock, 0, s>>>(a, nullptr));
     if (mark && hipEventRecord(mark, s) != hipSuccess) return 1;
-    k_fold<<<1, kFoldBlock, 0, s>>>(a.partials, g, a.result);
+    launch_fold(a.partials, g, a.result, s, a.done_flag, a.done_seq);
     return check_launch();
 }
 
@@ -2795,7 +2867,7 @@ int launch_step_multi(const StepArgs& a0, const MultiGrid& mg, hipStream_t s, hi
         rows += (unsigned)kWideGrid;
     }
     if (mark && hipEventRecord(mark, s) != hipSuccess) return 1;
-    k_fold<<<1, kFoldBlock, 0, s>>>(a.partials, rows, a.result);
+    launch_fold(a.partials, rows, a.result, s, a.done_flag, a.done_seq);
     return check_launch();
 }
 
@@ -2805,7 +2877,7 @@ int launch_step_direct(const StepArgs& a, hipStream_t s, hipEvent_t mark)
     const unsigned g = grid_for(a.g.N, kDirectBlock);
     RCP_DISPATCH_ND(a.g.ndim, k_step_direct, <<<g, kDirectBlock, 0, s>>>(a));
     if (mark && hipEventRecord(mark, s) != hipSuccess) return 1;
-    k_fold<<<1, kFoldBlock, 0, s>>>(a.partials, g, a.result);
+    launch_fold(a.partials, g, a.result, s, a.done_flag, a.done_seq);
     return check_launch();
 }
 
@@ -2816,7 +2888,7 @@ int launch_exact_step(const StepArgs& a, uint32_t* ref_count, uint32_t* ref_star
     const unsigned g = grid_for(a.g.N);
     RCP_DISPATCH_ND(a.g.ndim, k_step_list, <<<g, kBlock, 0, s>>>(a, ref_count));
     if (mark && hipEventRecord(mark, s) != hipSuccess) return 1;
-    k_fold<<<1, kFoldBlock, 0, s>>>(a.partials, g, a.result);
+    launch_fold(a.partials, g, a.result, s);
     if (launch_scan_u32(ref_count, ref_start, (size_t)a.g.N, scan_tmp, s)) return 1;
     k_exact_compact<<<g, kBlock, 0, s>>>(a, ref_start, ref_terms);
     RCP_DISPATCH_ND(a.g.ndim, k_exact_reduce, <<<1, kWave, 0, s>>>(a, ref_start, ref_terms));
@@ -3102,7 +3174,7 @@ int launch_mean_force(const StepArgs& a, const double* F, int exact, hipStream_t
 {
     if (a.g.N <= 0) return 0;
     k_mean_force<<<grid_for(a.g.N), kBlock, 0, s>>>(a, F);
-    k_fold<<<1, kFoldBlock, 0, s>>>(a.partials, grid_for(a.g.N), a.result);
+    launch_fold(a.partials, grid_for(a.g.N), a.result, s);
     if (exact) k_mean_force_exact<<<1, kWave, 0, s>>>(a);
     return check_launch();
 }
