@@ -212,11 +212,14 @@ int rcp_batch_set_progress(rcp_batch_progress_fn fn, void* user, uint64_t every)
  * their re-binning and step launches (ms). */
 int rcp_batch_last_stats(uint64_t* rounds, uint64_t* packing_steps, double* device_ms);
 
-/* Lanes per particle of the size-class step, process-wide: 0 (default) =
- * the library's rule (teams of 2, 4 or 8 lanes when a launch has too few
- * particles to fill the GPU one thread each); 1, 2, 4, 8 force that many
- * (teams stop at 4 lanes from five dimensions up).  Results do not depend
- * on it.  No reference counterpart (measurement hook for benchmarks). */
+/* Lanes per particle of the size-class and list steps, process-wide: 0
+ * (default) = the library's rule (teams of 2, 4 or 8 lanes when a launch has
+ * too few particles to fill the GPU one thread each); 1, 2, 4, 8 force that
+ * many (teams stop at 4 lanes from five dimensions up).  Size-class results
+ * do not depend on it; the list step's forces do not either, but its
+ * workgroup sums group 256 / lanes particles, so its fast-mode scalars
+ * round differently (within 1e-11 relative).  No reference counterpart
+ * (measurement hook for benchmarks). */
 int rcp_set_team_lanes(int lanes);
 
 /* ------------------------------------------------------- kernel level

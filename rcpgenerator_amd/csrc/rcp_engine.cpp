@@ -1042,7 +1042,10 @@ Engine::Scalars Engine::step_device(bool do_adam, double alpha, double inv_b1, d
                               mark))
             throw HipFailure("exact step launch");
     } else {
-        if (launch_step_list(a, stream_, mark)) throw HipFailure("list step launch");
+        const int lg = team_lanes_log2(N_, (int)nd_);
+        S.ensure_partials(list_partial_rows(N_, lg) + 1);
+        a.partials = S.partials;
+        if (launch_step_list(a, stream_, mark, lg)) throw HipFailure("list step launch");
     }
     if (opt_.time_kernels) timer_pending_[1] = true;   // mark was recorded after the kernel
     // the fold wrote the scalars to h_result

@@ -148,17 +148,24 @@ int launch_step_tile(const StepArgs& a, const TilePlan& p, hipStream_t s,
 size_t step_partial_blocks(size_t n);
 // Each step launcher runs the step kernel and then k_fold; when `mark` is
 // given it is recorded between the two (timing of the step kernel alone).
-// Lanes per particle of the size-class step (per-thread kernel or teams of
-// 2, 4 or 8 lanes; 4 at most from five dimensions up): the smallest team
-// that gives the launch kTeamThreads threads.  The figure is measured on
-// batched 4-D launches, where 22k, 56k and 141k particles ran fastest with
-// 8, 4 and 2 lanes.  Results do not depend on it.  set_team_lanes(n > 0)
+// Lanes per particle of the size-class and list steps (per-thread kernel
+// or teams of 2, 4 or 8 lanes; 4 at most from five dimensions up): the
+// smallest team that gives the launch kTeamThreads threads.  The figure is
+// measured on batched 4-D launches of the size-class step, where 22k, 56k
+// and 141k particles ran fastest with 8, 4 and 2 lanes.  Size-class results
+// do not depend on it; list-step forces neither (its fast-mode workgroup
+// sums group 256 >> lg particles).  set_team_lanes(n > 0)
 // forces n lanes (rounded down to a power of two within those limits; for
 // measurements), 0 restores the rule.
 constexpr size_t kTeamThreads = 160000;
 int team_lanes_log2(size_t particles, int ndim);
 void set_team_lanes(int lanes);
-int launch_step_list(const StepArgs& a, hipStream_t s, hipEvent_t mark = nullptr);
+// List step over CSR rows; team_lg > 0 takes 2^team_lg lanes per particle
+// (k_step_list_team: same forces and positions, partial rows of 256 >> lg
+// particles), which needs list_partial_rows(N, team_lg) partial rows.
+size_t list_partial_rows(size_t n, int team_lg);
+int launch_step_list(const StepArgs& a, hipStream_t s, hipEvent_t mark = nullptr,
+                     int team_lg = 0);
 int launch_step_direct(const StepArgs& a, hipStream_t s, hipEvent_t mark = nullptr);
 // List step whose scalar terms are then folded in the reference's
 // one-thread order (a.contrib: 4 per row slot, a.mag_out: N, a.inv_tag).

@@ -488,6 +488,97 @@ __global__ __launch_bounds__(kBlock) void k_step_list(StepArgs a, uint32_t* vcou
     finish_grid<kBlock>(a, blk);
 }
 
+// Team variant of the list step (fast reductions), for launches too small
+// to fill the chip with one thread per particle: 2^lg lanes of one
+// wavefront per particle.  Lane tl takes row entries tl, tl + 2^lg, ...;
+// in each round every lane computes its pair with pair_contact's arithmetic
+// and the team's first lane adds the round's pairs in row (tag) order, then
+// runs the particle's tail.  Forces and positions are k_step_list's bit for
+// bit; the workgroup sums group 256 >> lg particles instead of 256.
+template <int ND>
+__global__ __launch_bounds__(kBlock) void k_step_list_team(StepArgs a, int lg)
+{
+    const int lanes = 1 << lg;
+    const int tl = (int)(threadIdx.x & (unsigned)(lanes - 1));
+    const int lead = (int)(threadIdx.x % kWave) - tl;   // team's first lane in the wave
+    const int q = (int)((blockIdx.x * kBlock + threadIdx.x) >> lg);
+    double acc[S_NUM];
+#pragma unroll
+    for (int k = 0; k < S_NUM; ++k) acc[k] = 0.0;
+    if (q < a.g.N) {   // uniform across the team
+        Own<ND> o;
+        load_own<ND>(a, q, o);
+        double F[ND];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) F[d] = 0.0;
+        const uint32_t e0 = a.row_start[q], e1 = a.row_start[q + 1];
+        for (uint32_t base = e0; base < e1; base += (uint32_t)lanes) {
+            const uint32_t e = base + (uint32_t)tl;
+            bool hit = false;
+            int counted = 0;
+            double pf[ND], ov = 0.0, r = 0.0, dkt = 0.0, fm = 0.0;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) pf[d] = 0.0;
+            if (e < e1) {
+                const uint32_t qj = a.nbr[e];
+                double xj[ND], Dj, rkj;
+                load_partner<ND>(a, qj, xj, Dj, rkj);
+                const uint32_t tj = a.tag[qj];
+                r = 0.5 * (o.D + Dj);
+                double dl[ND];
+                double d2 = 0.0;
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    double delta = xj[d] - o.x[d];
+                    delta = mic(delta, a.g.inv_box[d], a.g.box[d]);
+                    dl[d] = delta;
+                    d2 += delta * delta;
+                }
+                hit = d2 < r * r;
+                if (hit) {
+                    const double dist = sqrt(d2);
+                    fm = -(r / dist - 1.0);
+#pragma unroll
+                    for (int d = 0; d < ND; ++d) pf[d] = fm * dl[d];
+                    if (tj > o.tag) {
+                        counted = 1;
+                        ov = 1.0 - dist / r;
+                        dkt = (a.use_rk ? (o.rk + rkj) : r) * (dist - r);
+                    }
+                }
+            }
+            const unsigned long long hits = __ballot(hit);
+            const uint32_t n = (e1 - base < (uint32_t)lanes) ? e1 - base : (uint32_t)lanes;
+            for (uint32_t u = 0; u < n; ++u) {   // uniform across the team
+                const int src = lead + (int)u;
+                if (!((hits >> src) & 1ull)) continue;
+                double pu[ND];
+#pragma unroll
+                for (int d = 0; d < ND; ++d) pu[d] = __shfl(pf[d], src, kWave);
+                const int cu = __shfl(counted, src, kWave);
+                const double ovu = __shfl(ov, src, kWave), ru = __shfl(r, src, kWave);
+                const double dku = __shfl(dkt, src, kWave), fmu = __shfl(fm, src, kWave);
+                if (tl == 0) {
+#pragma unroll
+                    for (int d = 0; d < ND; ++d) F[d] += pu[d];
+                    if (cu) {
+                        acc[S_U] += ovu;
+                        acc[S_CNT] += 1.0;
+                        acc[S_LC] += ru;
+                        acc[S_MAXOV] = (ovu > acc[S_MAXOV]) ? ovu : acc[S_MAXOV];
+                        acc[S_DK] = acc[S_DK] + dku;
+                        acc[S_FM] = acc[S_FM] + fmu;
+                    }
+                }
+            }
+        }
+        if (tl == 0) particle_tail<ND>(a, q, o, F, acc);
+    }
+    double blk[S_NUM];
+    block_reduce<kBlock>(acc, blk);
+    finish_grid<kBlock>(a, blk);
+}
+
 // ---------------------------------------------------------- direct step
 //
 // No neighbour list: thread q scans the 3^d stencil of its cell in the
@@ -2812,11 +2903,20 @@ int launch_step_tile(const StepArgs& a0, const TilePlan& p, hipStream_t s, hipEv
     return check_launch();
 }
 
-int launch_step_list(const StepArgs& a, hipStream_t s, hipEvent_t mark)
+size_t list_partial_rows(size_t n, int team_lg)
+{
+    return grid_for(n << team_lg);
+}
+
+int launch_step_list(const StepArgs& a, hipStream_t s, hipEvent_t mark, int team_lg)
 {
     if (a.g.N <= 0) return 0;
-    const unsigned g = grid_for(a.g.N);
-    RCP_DISPATCH_ND(a.g.ndim, k_step_list, <<<g, kBlock, 0, s>>>(a, nullptr));
+    const unsigned g = (unsigned)list_partial_rows((size_t)a.g.N, team_lg);
+    if (team_lg > 0) {
+        RCP_DISPATCH_ND(a.g.ndim, k_step_list_team, <<<g, kBlock, 0, s>>>(a, team_lg));
+    } else {
+        RCP_DISPATCH_ND(a.g.ndim, k_step_list, <<<g, kBlock, 0, s>>>(a, nullptr));
+    }
     if (mark && hipEventRecord(mark, s) != hipSuccess) return 1;
     launch_fold(a.partials, g, a.result, s, a.done_flag, a.done_seq);
     return check_launch();

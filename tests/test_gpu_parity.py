@@ -541,6 +541,36 @@ def test_team_lanes_do_not_change_results(gpu, nd, N, ratio):
         assert r["phi"] == ref["phi"] and r["max_min_dist"] == ref["max_min_dist"], lanes
 
 
+@pytest.mark.parametrize("nd,N,phi", [(2, 3000, 0.9), (3, 2000, 0.7), (5, 1500, 0.3)])
+def test_list_team_lanes_keep_positions(gpu, nd, N, phi):
+    """The list step with 1 (per-thread kernel), 2, 4 or 8 lanes per
+    particle: with the diameters fixed no scalar feeds back, so the
+    trajectories are bit-identical; the step scalars group the workgroup sums
+    differently and agree to 1e-11 relative.  Five dimensions stop at 4
+    lanes."""
+    from rcpgenerator_amd import capi
+    rng = np.random.default_rng(900 + nd)
+    x, D, box = random_config(rng, N, nd, phi)
+    out = {}
+    try:
+        for lanes in (1, 2, 4, 8):
+            capi.set_team_lanes(lanes)
+            e = gpu.Engine(x, D, box=box, max_steps=40, fix_diameter=True, neighbor_mode=1)
+            e.run(0)
+            out[lanes] = (e.result(), e.stats())
+    finally:
+        capi.set_team_lanes(0)
+    ref, st = out[1]
+    assert st["direct_steps"] == 0 and st["list_builds"] > 0
+    assert np.max(ref["max_overlap_history"]) > 0.0
+    for lanes in (2, 4, 8):
+        r, _ = out[lanes]
+        assert r["steps"] == ref["steps"] == 40, lanes
+        assert same(r["positions"], ref["positions"]), lanes
+        for key in ("energy_history", "force_history", "max_overlap_history"):
+            np.testing.assert_allclose(r[key], ref[key], rtol=1e-11, atol=0, err_msg=key)
+
+
 def test_team_lanes_batched_engine(gpu):
     """The batched engine's step with per-thread and 8-lane teams: identical
     results for every packing."""
