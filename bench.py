@@ -281,6 +281,9 @@ def bench_surrogate(args) -> int:
             "plain_engine_pool_value": plain_engine["engine_pool"],
             "vs_plain_engine_serial": out["engine_serial"]["guided_speedup_wall"],
             "vs_plain_engine_pool": out["engine_pool"]["guided_speedup_wall"],
+            "guided_pool_value": P / (out["guided_pool"]["wall_s"] / 3600.0),
+            "guided_pool_vs_plain_engine_pool": out["guided_pool"]["speedup_vs_engine_pool"],
+            "guided_pool_vs_plain_engine_serial": out["guided_pool"]["speedup_vs_engine_serial"],
             "dtype": "f64", "data": "synthetic", "evaluation": out,
             "config": {"workload": f"{P} held-out packings of the batched generator, N 500..2000, "
                                    "2-4 dimensions; guided batched runs vs plain batched runs "

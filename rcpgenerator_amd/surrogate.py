@@ -270,7 +270,8 @@ def evaluate(surr: "Surrogate", cases, threads: int = 8, log=print, detail: bool
     shortened schedule, the method without the network.  With engine_legs,
     the plain method also runs on the single-packing engine: one packing at
     a time (a loop over Packing.pack()) and as a pool of `threads` engines
-    running side by side on the device."""
+    running side by side on the device; and the guided schedules run on that
+    pool too."""
     inits = _initialize(cases, threads)
     plain, wall_p, _ = _batch(inits, [None] * len(inits), threads)
     X = np.stack([features_of_packing(D, c["Ndim"], list(w)) for c, (x, D, b, w) in
@@ -326,6 +327,28 @@ def evaluate(surr: "Surrogate", cases, threads: int = 8, log=print, detail: bool
         # then `threads` single engines at a time
         serial, wall_s, _ = _batch(inits, [None] * len(inits), 1, pool=True)
         pooled, wall_e, _ = _batch(inits, [None] * len(inits), threads, pool=True)
+        # the surrogate's schedules on the engine pool (no batch tail: each
+        # engine moves on to the next packing as soon as one finishes)
+        gpool, wall_gp, _ = _batch(inits, opts, threads, pool=True)
+        miss_p = [i for i in range(len(cases))
+                  if choice[i] >= 0 and abs(gpool[i]["phi"] - pred[i, 0]) > RERUN_TOL]
+        wall_gpr = 0.0
+        if miss_p:
+            again, wall_gpr, _ = _batch([inits[i] for i in miss_p], [None] * len(miss_p),
+                                        threads, pool=True)
+            for i, r in zip(miss_p, again):
+                gpool[i] = dict(r, steps=r["steps"] + gpool[i]["steps"])
+        phi_gp = np.array([r["phi"] for r in gpool])
+        out["guided_pool"] = {"wall_s": wall_gp + wall_gpr, "threads": threads,
+                              "reruns": len(miss_p),
+                              "steps": int(sum(r["steps"] for r in gpool)),
+                              "jammed": float(np.mean([r["max_min_dist"] < 5e-4 for r in gpool])),
+                              "phi_within_tol_vs_pool": float(np.mean(np.abs(
+                                  phi_gp - np.array([r["phi"] for r in pooled])) <= TOL)),
+                              "phi_mean_diff_vs_pool": float(np.mean(phi_gp) - np.mean(
+                                  [r["phi"] for r in pooled])),
+                              "speedup_vs_engine_pool": wall_e / (wall_gp + wall_gpr),
+                              "speedup_vs_engine_serial": wall_s / (wall_gp + wall_gpr)}
         for name, res, wall in (("engine_serial", serial, wall_s),
                                 ("engine_pool", pooled, wall_e)):
             phi_e = np.array([r["phi"] for r in res])
