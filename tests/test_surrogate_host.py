@@ -72,3 +72,63 @@ def test_committed_training_rows():
     # the labels restate the recorded schedule runs
     safe = np.abs(d["phi_sched"] - Y[:, :1]) <= S.TOL
     assert np.array_equal(safe.astype(np.float32), Y[:, 1:])
+
+
+def test_evaluation_reruns_misses_and_times_every_leg(monkeypatch):
+    """The evaluation protocol with the engines replaced by a table of
+    outcomes: a guided packing whose phi misses the prediction by more than
+    RERUN_TOL is run again with the default schedule and its steps count
+    both runs; packings the network calls unsafe take the default schedule;
+    the speed-ups are ratios of the legs' wall times."""
+    cases = [{"N": 1000, "Ndim": 3}, {"N": 1000, "Ndim": 3}, {"N": 1000, "Ndim": 2}]
+    inits = [(None, np.ones(1000), [1.0] * c["Ndim"], [0] * c["Ndim"]) for c in cases]
+    plain_phi = [0.640, 0.630, 0.850]
+    pred = np.array([[0.641, 0.9, 0.9, 0.9],     # schedule 0, lands on its prediction
+                     [0.629, 0.2, 0.8, 0.9],     # schedule 1, misses by 0.05: rerun
+                     [0.850, 0.1, 0.1, 0.1]],    # no schedule called safe: default
+                    dtype=np.float32)
+    walls = {}
+
+    def fake_batch(inits_, options, threads=8, pool=False):
+        key = ("pool" if pool else "batch", threads if pool else 0,
+               "plain" if all(o is None for o in options) else "sched")
+        res = []
+        for k, o in enumerate(options):
+            i = next(j for j, v in enumerate(inits) if v is inits_[k])
+            if o is None:
+                res.append({"phi": plain_phi[i], "steps": 1000, "max_min_dist": 1e-4})
+            else:
+                phi = pred[i, 0] - (0.05 if i == 1 else 0.0)
+                res.append({"phi": float(phi), "steps": 400, "max_min_dist": 1e-4})
+        # wall time: one unit per step in a batch, over the threads in a pool
+        wall = sum(r["steps"] for r in res) / (threads if pool else 1) / 1000.0
+        walls.setdefault(key, []).append(wall)
+        return res, wall, {}
+
+    class FakeSurrogate:
+        def predict(self, X):
+            assert X.shape == (3, S.N_FEATURES)
+            return pred
+
+    monkeypatch.setattr(S, "_initialize", lambda c, t: inits)
+    monkeypatch.setattr(S, "_batch", fake_batch)
+    out = S.evaluate(FakeSurrogate(), cases, threads=4, log=lambda m: None, detail=True,
+                     engine_legs=True)
+    assert out["choice"] == [0, 1, -1]
+    assert out["rerun_ids"] == [1] and out["reruns"] == 1
+    assert out["default"] == 1
+    assert out["steps_plain"] == 3000
+    # packing 0: 400; packing 1: 400 + 1000 (rerun); packing 2: 1000 (default)
+    assert out["steps_guided_each"] == [400, 1400, 1000]
+    assert out["phi_guided_each"][1] == plain_phi[1]       # the rerun's result is kept
+    assert out["speedup_steps"] == pytest.approx(3000 / 2800)
+    # guided batch (1.8 units) plus the rerun batch (1.0): 2.8 against 3.0 plain
+    assert out["wall_guided_s"] == pytest.approx(2.8)
+    assert out["speedup_wall"] == pytest.approx(3.0 / 2.8)
+    assert out["engine_serial"]["wall_s"] == pytest.approx(3.0)
+    assert out["engine_pool"]["wall_s"] == pytest.approx(0.75)
+    assert out["engine_pool"]["guided_speedup_wall"] == pytest.approx(0.75 / 2.8)
+    gp = out["guided_pool"]
+    assert gp["reruns"] == 1 and gp["steps"] == 2800
+    assert gp["wall_s"] == pytest.approx(1.8 / 4 + 1.0 / 4)
+    assert gp["speedup_vs_engine_serial"] == pytest.approx(3.0 / 0.7)
